@@ -1,0 +1,137 @@
+/*
+ * pdmd.h -- C ABI of the MI355X PRIME20 discontinuous-MD engine.
+ *
+ * Drop-in boundary for the reference's event-loop path
+ * (parallel-dmd-PRIME20/code/main.F90:484-1258 and the subroutines it
+ * calls).  Plain pointers and sizes only; every entry point returns an
+ * int status (0 = ok, < 0 = error; pdmd_last_error() gives the text).
+ * Bead, partner and event indices are 1-based as in the reference; arrays
+ * use the reference layouts (sv(6,N) column-major = x,y,z,vx,vy,vz per
+ * bead, positions in box units).  A context is not re-entrant: one host
+ * thread per context.
+ */
+#ifndef PDMD_H
+#define PDMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdmd_ctx pdmd_ctx;
+
+/* One PRIME20 system, as the reference reads it with -Drunr
+ * (inputinfo.f:28-411, main.F90:240-321): chains of chain_len[c] residues,
+ * per-residue side-chain identity restype (9..28, the G..V order of
+ * parameters/mass.data) and hp flag, box length in Angstrom, reduced
+ * temperature T* (stdin line 1 of temp_XXX), the .config/.lastvel state
+ * sv and the .bptnr table (1-based partner, 0 = none; NULL = no H-bonds).
+ * species[c] groups chains that share one sequence, the reference's
+ * nop1/nop2 blocks (NULL = every chain its own species).  seed keys the
+ * project RNG that replaces IFPORT drandm; canon != 0 enables Andersen
+ * ghost collisions (-Dcanon). */
+typedef struct {
+    int32_t nchains;
+    const int32_t* chain_len;
+    const int32_t* species;
+    const int32_t* restype;
+    const int32_t* hp;
+    double boxl;
+    double tstar;
+    uint64_t seed;
+    int32_t canon;
+    int32_t list_capacity; /* per-bead up/down list slots; 0 = default (96) */
+    const double* sv;
+    const int32_t* bptnr;
+} pdmd_system_desc;
+
+typedef struct {
+    int64_t coll;        /* committed calendar entries (main.F90:639) */
+    int64_t pair_events; /* bead-pair events only */
+    int64_t numghosts, nupdates, nforced;
+    int64_t nevents[31]; /* tally by resolved coltype (main.F90:926) */
+    double t, tfalse, interval, interval_max;
+    int32_t status, max_list;
+    int32_t nbeads, pad;
+} pdmd_stats_t;
+
+typedef struct {
+    int64_t coll;
+    int32_t i, j, type, pad;
+    double t; /* tfalse of the event */
+} pdmd_event_t;
+
+typedef struct {
+    double ered, tred, sumvel; /* energy.f:25-101 */
+    int32_t hb_alpha, hb_ii, hb_ij, pad;
+    double ehh_ii, ehh_ij;
+    double rg_avg;  /* radgyr.f:18-135 */
+    double e2e_avg; /* end2end.f:14-44 */
+} pdmd_obs_t;
+
+/* Library version string. */
+const char* pdmd_version(void);
+/* Text of the last error on this thread. */
+const char* pdmd_last_error(void);
+
+/* Context on the current HIP device; param_dir holds the reference's
+ * parameter tables (protein.data, beadwell_ha55a.data, rcarnrco.data,
+ * sqz6to10.data, mass.data, ep19p_ha55a_weakhp.data).
+ * Replaces inputinfo.f:141-398 table reading. */
+int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out);
+int pdmd_ctx_destroy(pdmd_ctx* ctx);
+
+/* Upload a system and run the reference's start-up on the device:
+ * scale_down.f, make_code.f (as a compact topology), the -Drunr H-bond
+ * restore (main.F90:240-321), nbor_setup.f, cell_link.f/nbor.f and
+ * events.f (main.F90:389-424).  *sys_id receives the system handle. */
+int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* desc, int32_t* sys_id);
+int pdmd_system_remove(pdmd_ctx* ctx, int32_t sys_id);
+int pdmd_nbeads(pdmd_ctx* ctx, int32_t sys_id);
+/* Device bytes a system of this description would occupy. */
+int64_t pdmd_system_bytes(const pdmd_system_desc* desc);
+
+/* Commit nevents[k] more calendar entries of each listed system, all
+ * systems concurrently in one launch (one wavefront per system), in the
+ * reference's serial event order.  Replaces the event loop
+ * main.F90:484-1258 (dispatch, commit, partial_events, calendar,
+ * ghost/sync/output pseudo-events).  Blocks until done. */
+int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* nevents);
+/* Asynchronous form for request loops: launch and return; pdmd_wait()
+ * blocks until the launch has finished. */
+int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* nevents);
+int pdmd_wait(pdmd_ctx* ctx);
+
+/* State in the reference layouts (any pointer may be NULL): sv(6,N),
+ * identity(N), bptnr(N), extra_repuls(N,4) as extra[4*(k-1)+m-1],
+ * tim/nptnr/coltype over N+3 calendar slots (header.f:24-46). */
+int pdmd_download(pdmd_ctx* ctx, int32_t sys_id, double* sv, int32_t* identity, int32_t* bptnr,
+                  int32_t* extra_repuls, double* tim, int32_t* nptnr, int32_t* coltype);
+int pdmd_stats(pdmd_ctx* ctx, int32_t sys_id, pdmd_stats_t* out);
+
+/* Event log of the next `capacity` committed events (coll, i, j, resolved
+ * type, event time); for parity checks against the CPU oracle. */
+int pdmd_log_enable(pdmd_ctx* ctx, int32_t sys_id, int32_t capacity);
+int pdmd_log_read(pdmd_ctx* ctx, int32_t sys_id, int32_t max, pdmd_event_t* out, int32_t* n);
+
+/* Test hook replacing core.f/bond.f/sqwel.f/sqshlder.f/nc_sqwel.f called
+ * as events.f:28-47 does: tij (1e9 when none) and type for pairs (i, j),
+ * i < j, at the system's current tfalse. */
+int pdmd_predict_pairs(pdmd_ctx* ctx, int32_t sys_id, int32_t n, const int32_t* i, const int32_t* j, double* tij,
+                       int32_t* type);
+
+/* Observables of the output pseudo-event (energy.f, radgyr.f, end2end.f). */
+int pdmd_observables(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
+
+/* Seeded synthetic input (no reference counterpart: the shipped generator
+ * genconfig/gen_config_random-SQZ.f90 is template-based and IFPORT-seeded):
+ * random sequences, self-avoiding coils satisfying every bond window and
+ * core, Maxwell-Boltzmann velocities at T*, box from rho* = N sigma_1^3/V.
+ * restype/hp hold sum(chain_len) entries, sv 6*N. */
+int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_len, double density, double tstar,
+                  uint64_t seed, int32_t* restype, int32_t* hp, double* sv, double* boxl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

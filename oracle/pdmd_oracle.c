@@ -1563,7 +1563,8 @@ int64_t orc_run(orc_sys* S, int64_t nev, orc_event* log, int64_t log_cap) {
         for (int k = S->tlinks[best]; k != 0; k = S->tlinks[k])
             if (S->tim[k] < S->tim[best]) best = k;
         int i = best;
-        S->tfalse = S->tim[i];
+        const double tsel = S->tim[i];
+        S->tfalse = tsel;
         S->coll++;
         int j = 0, type = 0;
         if (i <= N) {
@@ -1582,7 +1583,7 @@ int64_t orc_run(orc_sys* S, int64_t nev, orc_event* log, int64_t log_cap) {
             log[done].j = j;
             log[done].type = type;
             log[done].pad = 0;
-            log[done].t = S->tfalse;
+            log[done].t = tsel;
         }
         done++;
         if (S->err) return S->err;
@@ -1986,7 +1987,7 @@ int32_t orc_nbeads(const orc_sys* S) { return S->N; }
 void orc_get_state(const orc_sys* S, double* sv, int32_t* identity, int32_t* bptnr,
                    int32_t* extra_repuls, double* tim, int32_t* nptnr, int32_t* coltype) {
     int N = S->N;
-    if (sv) memcpy(sv, S->sv + 6, sizeof(double) * 6 * (size_t)N);
+    if (sv) memcpy(sv, S->sv, sizeof(double) * 6 * (size_t)N);
     for (int k = 1; k <= N; ++k) {
         if (identity) identity[k - 1] = S->identity[k];
         if (bptnr) bptnr[k - 1] = S->bptnr[k];

@@ -1,0 +1,794 @@
+// pdmd_capi.cpp -- C ABI of the MI355X PRIME20 DMD engine (include/pdmd.h):
+// contexts, system upload with the reference's start-up sequence, device
+// launches, downloads, event logs and observables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pdmd.h"
+#include "pdmd_device.h"
+#include "pdmd_engine.h"
+#include "pdmd_host.h"
+
+namespace pdmd {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg, int code = -1) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return fail(std::string(#expr ": ") + hipGetErrorString(e_), -2); \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count) {
+        n = count;
+        return hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
+    }
+};
+
+struct HostSys {
+    Topology topo;
+    Params p;  // scaled to this box
+    Cutoffs cut{};
+    int N = 0, cap = 96, canon = 1;
+    double boxl = 0, tstar = 0;
+    uint64_t seed = 0;
+    // device memory
+    DevBuf<DevTables> tab;
+    DevBuf<Bead> bead;
+    DevBuf<int> ident, bptnr, chain, loc, clen, res, restype, hp, nb_up, nup, nb_dn, ndn, nptnr, coltype, cbin,
+        cfine, ckey, cstart, crun, csorted;
+    DevBuf<int4> extra, ovr;
+    DevBuf<double> bondw, tim;
+    DevBuf<double4> old_r;
+    DevBuf<SysState> st;
+    DevBuf<LogRec> log;
+    SysDev dev{};
+    bool alive = false;
+};
+
+}  // namespace
+}  // namespace pdmd
+
+struct pdmd_ctx {
+    pdmd::Params raw;
+    std::string param_dir;
+    std::vector<std::unique_ptr<pdmd::HostSys>> sys;
+    pdmd::DevBuf<pdmd::SysDev> d_sys;
+    pdmd::DevBuf<int> d_ids;
+    pdmd::DevBuf<long long> d_inc;
+    hipStream_t stream = nullptr;
+    int device = 0;
+};
+
+namespace pdmd {
+namespace {
+
+// ev_code writes of the host-side start-up (same slot semantics as the
+// device): an override is kept only where it differs from the static code.
+struct HostCodes {
+    const Topology& t;
+    std::vector<int4>& ovr;
+    int dyn(int row, int col) const {
+        const int4 o = ovr[row];
+        const int s[4] = {o.x, o.y, o.z, o.w};
+        for (int k = 0; k < 4; ++k)
+            if (s[k] >= 0 && (s[k] >> 8) == col) return s[k] & 0xff;
+        return static_code(t, row, col);
+    }
+    bool set(int row, int col, int code) {
+        int4& o = ovr[row];
+        int* s = &o.x;
+        for (int k = 0; k < 4; ++k)
+            if (s[k] >= 0 && (s[k] >> 8) == col) s[k] = -1;
+        if (code != static_code(t, row, col)) {
+            int k = 0;
+            while (k < 4 && s[k] >= 0) ++k;
+            if (k == 4) return false;
+            s[k] = (col << 8) | code;
+        }
+        return true;
+    }
+};
+
+bool host_repuls_add(const Topology& t, HostCodes& hc, std::vector<int4>& extra, int i, int j) {
+    const int Li = t.chain_len[t.chain[i]], Lj = t.chain_len[t.chain[j]];
+    const int ncim1 = i + Li - 1, ncai = i - Li, ncaj = j - 2 * Lj, nnjp1 = j - Lj + 1;
+    bool ok = hc.set(ncaj, i, 50) && hc.set(i, ncaj, 40) && hc.set(i, nnjp1, 40) && hc.set(nnjp1, i, 50) &&
+              hc.set(ncai, j, 50) && hc.set(j, ncai, 40) && hc.set(j, ncim1, 40) && hc.set(ncim1, j, 50);
+    extra[i].x = ncaj;
+    extra[i].y = nnjp1;
+    extra[ncaj].z = i;
+    extra[nnjp1].z = i;
+    extra[j].x = ncai;
+    extra[j].y = ncim1;
+    extra[ncai].z = j;
+    extra[ncim1].z = j;
+    extra[i].w = j;
+    extra[j].w = i;
+    return ok;
+}
+
+bool terminal_bead(const Topology& t, int k) {
+    const int L = t.chain_len[t.chain[k]];
+    return t.loc[k] == L + 1 || t.loc[k] == 3 * L;
+}
+
+template <class T>
+hipError_t upload(DevBuf<T>& b, const std::vector<T>& v) {
+    hipError_t e = b.alloc(v.size());
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+    return e;
+}
+
+int validate_desc(const pdmd_system_desc* d) {
+    if (!d || d->nchains <= 0 || !d->chain_len || !d->restype || !d->hp || !d->sv)
+        return fail("pdmd_system_desc: missing arrays");
+    for (int c = 0; c < d->nchains; ++c)
+        if (d->chain_len[c] < 4) return fail("pdmd_system_desc: chains need at least 4 residues");
+    if (!(d->boxl > 0) || !(d->tstar > 0)) return fail("pdmd_system_desc: boxl and tstar must be positive");
+    return 0;
+}
+
+}  // namespace
+}  // namespace pdmd
+
+using namespace pdmd;
+
+extern "C" {
+
+const char* pdmd_version(void) { return "pdmd-mi355x 0.1 (gfx950)"; }
+const char* pdmd_last_error(void) { return g_err.c_str(); }
+
+int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
+    if (!out || !param_dir) return fail("pdmd_ctx_create: null argument");
+    auto ctx = std::make_unique<pdmd_ctx>();
+    std::string err;
+    if (!load_params(ctx->raw, param_dir, err)) return fail(err);
+    ctx->param_dir = param_dir;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device", -3);
+    HIPCHK(hipGetDevice(&ctx->device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+    return 0;
+}
+
+int pdmd_ctx_destroy(pdmd_ctx* ctx) {
+    if (!ctx) return 0;
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return 0;
+}
+
+int64_t pdmd_system_bytes(const pdmd_system_desc* d) {
+    if (validate_desc(d)) return -1;
+    int64_t N = 0, nres = 0;
+    for (int c = 0; c < d->nchains; ++c) {
+        N += 4 * d->chain_len[c];
+        nres += d->chain_len[c];
+    }
+    const int64_t cap = d->list_capacity > 0 ? d->list_capacity : 96;
+    int64_t b = N * (int64_t)sizeof(Bead) + N * 4 * 16 /*ints*/ + 2 * N * 16 /*extra, ovr*/ + N * 32 /*old_r*/ +
+                2 * N * cap * 4 + 2 * N * 4 + (N + 3) * 20 + nres * 48 + (int64_t)sizeof(DevTables);
+    // coarse grid upper bound: box/rl cells per axis at the densest tables
+    b += 4 * N * 4 + 2 * 4 * 1024 * 1024;
+    return b;
+}
+
+int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
+    if (!ctx || !sys_id) return fail("pdmd_system_add: null argument");
+    if (int e = validate_desc(d)) return e;
+    auto hs = std::make_unique<HostSys>();
+    HostSys& H = *hs;
+    const int nc = d->nchains;
+    std::vector<int> clen(d->chain_len, d->chain_len + nc), spec(nc);
+    for (int c = 0; c < nc; ++c) spec[c] = d->species ? d->species[c] : c;
+    int nres = 0;
+    for (int c = 0; c < nc; ++c) nres += clen[c];
+    std::vector<int> rt(d->restype, d->restype + nres), hp(d->hp, d->hp + nres);
+    for (int q = 0; q < nres; ++q)
+        if (rt[q] < 9 || rt[q] > 28) return fail("pdmd_system_add: restype must be 9..28");
+    H.topo.build(clen, spec, rt, hp);
+    const Topology& t = H.topo;
+    const int N = t.N;
+    H.N = N;
+    H.cap = d->list_capacity > 0 ? d->list_capacity : 96;
+    if (H.cap > 255) return fail("pdmd_system_add: list_capacity above 255");
+    if (N >= (1 << 24)) return fail("pdmd_system_add: too many beads");
+    H.canon = d->canon ? 1 : 0;
+    H.boxl = d->boxl;
+    H.tstar = d->tstar;
+    H.seed = d->seed;
+    H.p = ctx->raw;
+    scale_params(H.p, d->boxl);
+    const Params& P = H.p;
+    // state: positions wrapped as inputinfo.f:90-95 and main.F90:210-216 do
+    std::vector<Bead> bead(N);
+    for (int k = 0; k < N; ++k) {
+        Bead b{};
+        double x = d->sv[6 * k], y = d->sv[6 * k + 1], z = d->sv[6 * k + 2];
+        x = x - std::round(x);
+        y = y - std::round(y);
+        z = z - std::round(z);
+        x = x - std::round(x);
+        y = y - std::round(y);
+        z = z - std::round(z);
+        b.x = x;
+        b.y = y;
+        b.z = z;
+        b.vx = d->sv[6 * k + 3];
+        b.vy = d->sv[6 * k + 4];
+        b.vz = d->sv[6 * k + 5];
+        b.bm = ctx->raw.bmass[t.ident0[k]];  // inputinfo.f:400-403, initial identities
+        bead[k] = b;
+    }
+    std::vector<int> ident = t.ident0, bptnr(N, -1);
+    std::vector<int4> extra(N, make_int4(-1, -1, -1, -1)), ovr(N, make_int4(-1, -1, -1, -1));
+    if (d->bptnr)
+        for (int k = 0; k < N; ++k) {
+            const int b = d->bptnr[k];
+            if (b < 0 || b > N) return fail("pdmd_system_add: bptnr out of range");
+            bptnr[k] = b - 1;
+        }
+    // -Drunr restore of H-bond identities and shoulders, main.F90:240-321:
+    // pair loop k < kj restricted to the pairs either branch can act on
+    HostCodes hc{t, ovr};
+    std::vector<int> nbeads, cbeads;
+    for (int k = 0; k < N; ++k) {
+        const int L = t.chain_len[t.chain[k]], kind = (t.loc[k] - 1) / L;
+        if (kind == 1) nbeads.push_back(k);
+        if (kind == 2) cbeads.push_back(k);
+    }
+    for (int k = 0; k < N; ++k) {
+        const int L = t.chain_len[t.chain[k]], kind = (t.loc[k] - 1) / L;
+        std::vector<int> cand;
+        if (kind == 1 || kind == 2) {
+            const std::vector<int>& other = kind == 1 ? cbeads : nbeads;
+            for (int q : other)
+                if (q > k) cand.push_back(q);
+        }
+        if (bptnr[k] > k && (cand.empty() || std::find(cand.begin(), cand.end(), bptnr[k]) == cand.end())) {
+            cand.push_back(bptnr[k]);
+            std::sort(cand.begin(), cand.end());
+        }
+        for (int kj : cand) {
+            if (ident[k] + ident[kj] == 5 && hc.dyn(k, kj) == 15) {
+                double rx = bead[k].x - bead[kj].x, ry = bead[k].y - bead[kj].y, rz = bead[k].z - bead[kj].z;
+                rx = rx - std::round(rx);
+                ry = ry - std::round(ry);
+                rz = rz - std::round(rz);
+                const double r2 = rx * rx + ry * ry + rz * rz;
+                const double diff = r2 - P.welldia_sq[ident[k]][ident[kj]];
+                if (diff < 0.0 && !terminal_bead(t, k) && !terminal_bead(t, kj)) {
+                    const bool ok = ident[k] == 1 ? host_repuls_add(t, hc, extra, k, kj)
+                                                  : host_repuls_add(t, hc, extra, kj, k);
+                    if (!ok) return fail("pdmd_system_add: override slots exhausted in H-bond restore");
+                }
+            }
+            if (kj == bptnr[k]) {
+                if (ident[k] == 1) {
+                    ident[k] = 5;
+                    ident[kj] = 8;
+                } else {
+                    ident[k] = 8;
+                    ident[kj] = 5;
+                }
+            }
+        }
+    }
+    nbor_setup(P, t, ident, H.cut);
+    const int M = (int)(1.0 / (H.cut.sig_max_all * kRlConst) * kNWrap);  // main.F90:390, interior cells
+    if (M < 2 * kNWrap + 1) return fail("pdmd_system_add: box too small for the n_wrap=2 cell stencil");
+    if (M >= 1024) return fail("pdmd_system_add: box too large for packed cell coordinates");
+    const int Mq = M / 2;
+    // device tables
+    DevTables tab{};
+    for (int a = 0; a < 29; ++a)
+        for (int b = 0; b < 29; ++b) {
+            tab.sigma_sq[a][b] = P.sigma_sq[a][b];
+            tab.welldia_sq[a][b] = P.welldia_sq[a][b];
+            tab.ep_sqrt[a][b] = P.ep_sqrt[a][b];
+            tab.shlddia_sq[a][b] = P.shlddia_sq[a][b];
+        }
+    std::memcpy(tab.ev_param, P.ev_param, sizeof tab.ev_param);
+    std::memcpy(tab.sqz610, P.sqz610, sizeof tab.sqz610);
+    std::memcpy(tab.rlsq, H.cut.rlsq, sizeof tab.rlsq);
+    for (int k = 0; k < 5; ++k) tab.shder[k] = P.shder_dist[k];
+    tab.epsilon1 = P.epsilon[1];
+    // bond windows per residue (inputinfo.f:300-349, bond.f:21-31)
+    std::vector<double> bondw(6 * (size_t)nres);
+    for (int q = 0; q < nres; ++q) {
+        const int iii = rt[q] - 8;
+        const double bd = ctx->raw.drca[iii] / P.boxl_orig, brn = ctx->raw.drnh[iii] / P.boxl_orig,
+                     brc = ctx->raw.drco[iii] / P.boxl_orig;
+        const double dd = ctx->raw.del_rca[iii], dn = ctx->raw.del_rnh[iii], dc = ctx->raw.del_rco[iii];
+        bondw[6 * q + 0] = (1.0 - dd) * bd;
+        bondw[6 * q + 1] = (1.0 + dd) * bd;
+        bondw[6 * q + 2] = (1.0 - dn) * brn;
+        bondw[6 * q + 3] = (1.0 + dn) * brn;
+        bondw[6 * q + 4] = (1.0 - dc) * brc;
+        bondw[6 * q + 5] = (1.0 + dc) * brc;
+    }
+    std::vector<int> clen_b(N);
+    for (int k = 0; k < N; ++k) clen_b[k] = t.chain_len[t.chain[k]];
+    SysState st{};
+    const double setemp = d->tstar * 12.0;  // main.F90:126-127
+    st.t_fact = 0.00005;
+    st.n_forced = 150.0;
+    st.interval = st.t_fact / std::sqrt(setemp);
+    st.interval_max = st.n_forced * st.interval;
+    st.sortsize = st.interval_max / (double)kNumbin;
+    st.nbin = 1;
+    // uploads
+    {
+        std::vector<DevTables> tv(1, tab);
+        HIPCHK(upload(H.tab, tv));
+    }
+    HIPCHK(upload(H.bead, bead));
+    HIPCHK(upload(H.ident, ident));
+    HIPCHK(upload(H.bptnr, bptnr));
+    HIPCHK(upload(H.extra, extra));
+    HIPCHK(upload(H.ovr, ovr));
+    HIPCHK(upload(H.chain, t.chain));
+    HIPCHK(upload(H.loc, t.loc));
+    HIPCHK(upload(H.clen, clen_b));
+    HIPCHK(upload(H.res, t.res));
+    HIPCHK(upload(H.restype, rt));
+    HIPCHK(upload(H.hp, hp));
+    HIPCHK(upload(H.bondw, bondw));
+    HIPCHK(H.old_r.alloc(N));
+    HIPCHK(H.nb_up.alloc((size_t)N * H.cap));
+    HIPCHK(H.nb_dn.alloc((size_t)N * H.cap));
+    HIPCHK(H.nup.alloc(N));
+    HIPCHK(H.ndn.alloc(N));
+    HIPCHK(H.tim.alloc(N + 3));
+    HIPCHK(H.nptnr.alloc(N + 3));
+    HIPCHK(H.coltype.alloc(N + 3));
+    HIPCHK(H.cbin.alloc(N + 3));
+    HIPCHK(H.cfine.alloc(N));
+    HIPCHK(H.ckey.alloc(N));
+    HIPCHK(H.csorted.alloc(N));
+    HIPCHK(H.cstart.alloc((size_t)Mq * Mq * Mq + 1));
+    HIPCHK(H.crun.alloc((size_t)Mq * Mq * Mq));
+    {
+        std::vector<SysState> sv(1, st);
+        HIPCHK(upload(H.st, sv));
+    }
+    SysDev& D = H.dev;
+    D.N = N;
+    D.nres = nres;
+    D.cap = H.cap;
+    D.canon = H.canon;
+    D.M = M;
+    D.Mq = Mq;
+    D.ncoarse = Mq * Mq * Mq;
+    D.log_cap = 0;
+    D.width = 1.0 / (double)M;
+    D.half = 1.0 / 2.0;
+    D.setemp = setemp;
+    D.avegtime = 0.00005 / std::sqrt(setemp);
+    D.hdelr = H.cut.hdelr;
+    D.seed = d->seed;
+    D.tab = H.tab.p;
+    D.bead = H.bead.p;
+    D.ident = H.ident.p;
+    D.bptnr = H.bptnr.p;
+    D.extra = H.extra.p;
+    D.ovr = H.ovr.p;
+    D.chain = H.chain.p;
+    D.loc = H.loc.p;
+    D.clen = H.clen.p;
+    D.res = H.res.p;
+    D.restype = H.restype.p;
+    D.hp = H.hp.p;
+    D.bondw = H.bondw.p;
+    D.old_r = H.old_r.p;
+    D.nb_up = H.nb_up.p;
+    D.nup = H.nup.p;
+    D.nb_dn = H.nb_dn.p;
+    D.ndn = H.ndn.p;
+    D.tim = H.tim.p;
+    D.nptnr = H.nptnr.p;
+    D.coltype = H.coltype.p;
+    D.cbin = H.cbin.p;
+    D.cfine = H.cfine.p;
+    D.ckey = H.ckey.p;
+    D.cstart = H.cstart.p;
+    D.crun = H.crun.p;
+    D.csorted = H.csorted.p;
+    D.st = H.st.p;
+    D.log = nullptr;
+    H.alive = true;
+    // slot
+    int id = -1;
+    for (size_t k = 0; k < ctx->sys.size(); ++k)
+        if (!ctx->sys[k]) {
+            id = (int)k;
+            break;
+        }
+    if (id < 0) {
+        id = (int)ctx->sys.size();
+        ctx->sys.emplace_back();
+    }
+    ctx->sys[id] = std::move(hs);
+    // device descriptor table
+    if (ctx->d_sys.n < ctx->sys.size()) {
+        DevBuf<SysDev> nb;
+        HIPCHK(nb.alloc(ctx->sys.size() + 16));
+        if (ctx->d_sys.p) HIPCHK(hipMemcpy(nb.p, ctx->d_sys.p, sizeof(SysDev) * ctx->d_sys.n, hipMemcpyDeviceToDevice));
+        std::swap(ctx->d_sys.p, nb.p);
+        std::swap(ctx->d_sys.n, nb.n);
+    }
+    HIPCHK(hipMemcpy(ctx->d_sys.p + id, &ctx->sys[id]->dev, sizeof(SysDev), hipMemcpyHostToDevice));
+    // device start-up: lists, pseudo-event seeds, events()
+    if (ctx->d_ids.n < 1) HIPCHK(ctx->d_ids.alloc(4096));
+    if (ctx->d_inc.n < 1) HIPCHK(ctx->d_inc.alloc(4096));
+    HIPCHK(hipMemcpy(ctx->d_ids.p, &id, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, 2 * H.cap + 8, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    SysState chk;
+    HIPCHK(hipMemcpy(&chk, H.st.p, sizeof chk, hipMemcpyDeviceToHost));
+    if (chk.status) {
+        ctx->sys[id].reset();
+        return fail("pdmd_system_add: device start-up failed with status " + std::to_string(chk.status), -4);
+    }
+    *sys_id = id + 1;
+    return 0;
+}
+
+static HostSys* get_sys(pdmd_ctx* ctx, int32_t id) {
+    if (!ctx || id < 1 || id > (int)ctx->sys.size() || !ctx->sys[id - 1]) {
+        fail("invalid system id");
+        return nullptr;
+    }
+    return ctx->sys[id - 1].get();
+}
+
+int pdmd_system_remove(pdmd_ctx* ctx, int32_t id) {
+    if (!get_sys(ctx, id)) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->sys[id - 1].reset();
+    return 0;
+}
+
+int pdmd_nbeads(pdmd_ctx* ctx, int32_t id) {
+    HostSys* H = get_sys(ctx, id);
+    return H ? H->N : -1;
+}
+
+int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev) {
+    if (!ctx || nsys <= 0 || !ids || !nev) return fail("pdmd_advance: bad arguments");
+    if (nsys > 4096) return fail("pdmd_advance: at most 4096 systems per launch");
+    std::vector<int> dids(nsys);
+    std::vector<long long> inc(nsys);
+    int max_n = 0, max_cap = 0;
+    for (int k = 0; k < nsys; ++k) {
+        HostSys* H = get_sys(ctx, ids[k]);
+        if (!H) return -1;
+        for (int q = 0; q < k; ++q)
+            if (ids[q] == ids[k]) return fail("pdmd_advance: duplicate system id");
+        if (nev[k] < 0) return fail("pdmd_advance: negative event count");
+        dids[k] = ids[k] - 1;
+        inc[k] = nev[k];
+        max_n = std::max(max_n, H->N);
+        max_cap = std::max(max_cap, H->cap);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_ids.p, dids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_inc.p, inc.data(), sizeof(long long) * nsys, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, 2 * max_cap + 8, ctx->stream));
+    // the copies above read pageable memory synchronously before returning
+    return 0;
+}
+
+int pdmd_wait(pdmd_ctx* ctx) {
+    if (!ctx) return fail("pdmd_wait: null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev) {
+    int e = pdmd_advance_async(ctx, nsys, ids, nev);
+    if (e) return e;
+    e = pdmd_wait(ctx);
+    if (e) return e;
+    for (int k = 0; k < nsys; ++k) {
+        HostSys* H = get_sys(ctx, ids[k]);
+        int status = 0;
+        HIPCHK(hipMemcpy(&status, &H->st.p->status, sizeof(int), hipMemcpyDeviceToHost));
+        if (status) return fail("system " + std::to_string(ids[k]) + " stopped with device status " + std::to_string(status), -10 - status);
+    }
+    return 0;
+}
+
+int pdmd_download(pdmd_ctx* ctx, int32_t id, double* sv, int32_t* identity, int32_t* bptnr, int32_t* extra_repuls,
+                  double* tim, int32_t* nptnr, int32_t* coltype) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int N = H->N;
+    if (sv) {
+        std::vector<Bead> b(N);
+        HIPCHK(hipMemcpy(b.data(), H->bead.p, sizeof(Bead) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) {
+            sv[6 * k] = b[k].x;
+            sv[6 * k + 1] = b[k].y;
+            sv[6 * k + 2] = b[k].z;
+            sv[6 * k + 3] = b[k].vx;
+            sv[6 * k + 4] = b[k].vy;
+            sv[6 * k + 5] = b[k].vz;
+        }
+    }
+    if (identity) HIPCHK(hipMemcpy(identity, H->ident.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+    if (bptnr) {
+        HIPCHK(hipMemcpy(bptnr, H->bptnr.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) bptnr[k] = bptnr[k] + 1;
+    }
+    if (extra_repuls) {
+        std::vector<int4> e(N);
+        HIPCHK(hipMemcpy(e.data(), H->extra.p, sizeof(int4) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) {
+            extra_repuls[4 * k] = e[k].x + 1;
+            extra_repuls[4 * k + 1] = e[k].y + 1;
+            extra_repuls[4 * k + 2] = e[k].z + 1;
+            extra_repuls[4 * k + 3] = e[k].w + 1;
+        }
+    }
+    if (tim) HIPCHK(hipMemcpy(tim, H->tim.p, sizeof(double) * (N + 3), hipMemcpyDeviceToHost));
+    if (nptnr) {
+        HIPCHK(hipMemcpy(nptnr, H->nptnr.p, sizeof(int) * (N + 3), hipMemcpyDeviceToHost));
+        for (int k = 0; k < N + 3; ++k)
+            if (nptnr[k] >= 0) nptnr[k] = nptnr[k] + 1;
+    }
+    if (coltype) HIPCHK(hipMemcpy(coltype, H->coltype.p, sizeof(int) * (N + 3), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pdmd_stats(pdmd_ctx* ctx, int32_t id, pdmd_stats_t* out) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H || !out) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    SysState st;
+    HIPCHK(hipMemcpy(&st, H->st.p, sizeof st, hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof *out);
+    out->coll = st.coll;
+    long long pe = 0;
+    for (int k = 1; k <= 30; ++k) {
+        out->nevents[k] = st.nevents[k];
+        pe += st.nevents[k];
+    }
+    out->pair_events = pe;
+    out->numghosts = st.numghosts;
+    out->nupdates = st.nupdates;
+    out->nforced = st.nforced;
+    out->t = st.t;
+    out->tfalse = st.tfalse;
+    out->interval = st.interval;
+    out->interval_max = st.interval_max;
+    out->status = st.status;
+    out->max_list = (int32_t)st.max_nb_seen;
+    out->nbeads = H->N;
+    return 0;
+}
+
+int pdmd_log_enable(pdmd_ctx* ctx, int32_t id, int32_t capacity) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (capacity < 0) return fail("pdmd_log_enable: negative capacity");
+    if (H->log.p) {
+        (void)hipFree(H->log.p);
+        H->log.p = nullptr;
+        H->log.n = 0;
+    }
+    if (capacity > 0) HIPCHK(H->log.alloc(capacity));
+    H->dev.log = capacity > 0 ? H->log.p : nullptr;
+    H->dev.log_cap = capacity;
+    int zero = 0;
+    HIPCHK(hipMemcpy(&H->st.p->log_count, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_sys.p + (id - 1), &H->dev, sizeof(SysDev), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pdmd_log_read(pdmd_ctx* ctx, int32_t id, int32_t max, pdmd_event_t* out, int32_t* n) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H || !out || !n) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int cnt = 0;
+    HIPCHK(hipMemcpy(&cnt, &H->st.p->log_count, sizeof(int), hipMemcpyDeviceToHost));
+    cnt = std::min(cnt, H->dev.log_cap);
+    cnt = std::min(cnt, max);
+    if (cnt > 0) {
+        std::vector<LogRec> r(cnt);
+        HIPCHK(hipMemcpy(r.data(), H->log.p, sizeof(LogRec) * cnt, hipMemcpyDeviceToHost));
+        for (int k = 0; k < cnt; ++k) {
+            out[k].coll = r[k].coll;
+            out[k].i = r[k].i;
+            out[k].j = r[k].j;
+            out[k].type = r[k].type;
+            out[k].pad = 0;
+            out[k].t = r[k].t;
+        }
+    }
+    *n = cnt;
+    return 0;
+}
+
+int pdmd_predict_pairs(pdmd_ctx* ctx, int32_t id, int32_t n, const int32_t* pi, const int32_t* pj, double* tij,
+                       int32_t* type) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H || n < 0 || (n > 0 && (!pi || !pj || !tij || !type))) return fail("pdmd_predict_pairs: bad arguments");
+    if (n == 0) return 0;
+    for (int k = 0; k < n; ++k)
+        if (pi[k] < 1 || pj[k] < 1 || pi[k] > H->N || pj[k] > H->N || pi[k] == pj[k])
+            return fail("pdmd_predict_pairs: pair index out of range");
+    DevBuf<int> di, dj, dt;
+    DevBuf<double> dtij;
+    HIPCHK(di.alloc(n));
+    HIPCHK(dj.alloc(n));
+    HIPCHK(dt.alloc(n));
+    HIPCHK(dtij.alloc(n));
+    HIPCHK(hipMemcpy(di.p, pi, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj.p, pj, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(engine_predict(ctx->d_sys.p, id - 1, n, di.p, dj.p, dtij.p, dt.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(tij, dtij.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(type, dt.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H || !o) return -1;
+    const int N = H->N;
+    std::vector<double> sv(6 * (size_t)N);
+    std::vector<int> ident(N), bp(N);
+    if (int e = pdmd_download(ctx, id, sv.data(), ident.data(), bp.data(), nullptr, nullptr, nullptr, nullptr)) return e;
+    SysState st;
+    HIPCHK(hipMemcpy(&st, H->st.p, sizeof st, hipMemcpyDeviceToHost));
+    const double tf = st.tfalse;
+    const Topology& t = H->topo;
+    const Params& P = H->p;
+    std::memset(o, 0, sizeof *o);
+    // energy.f:25-101
+    int hb_ii = 0, hb_ij = 0, hb_alpha = 0;
+    for (int i = 0; i < N; ++i) {
+        const int j = bp[i] - 1;
+        if (j > i) (t.chain[i] == t.chain[j] ? hb_ii : hb_ij)++;
+    }
+    double ehh_ii = 0.0, ehh_ij = 0.0;
+    auto pos = [&](int k, int c) { return sv[6 * k + c] + sv[6 * k + 3 + c] * tf; };
+    std::vector<int> side;
+    for (int k = 0; k < N; ++k)
+        if ((t.loc[k] - 1) / t.chain_len[t.chain[k]] == 3) side.push_back(k);
+    for (size_t a = 0; a < side.size(); ++a)
+        for (size_t b = a + 1; b < side.size(); ++b) {
+            const int i = side[a], j = side[b];
+            if (bp[i] - 1 == j) continue;
+            if (static_code(t, j, i) != 16) continue;
+            double r[3];
+            for (int c = 0; c < 3; ++c) {
+                const double v = sv[6 * i + 3 + c] - sv[6 * j + 3 + c];
+                double x = sv[6 * i + c] - sv[6 * j + c] + v * tf;
+                r[c] = x - std::round(x);
+            }
+            const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            if (r2 <= P.welldia_sq[ident[i]][ident[j]]) {
+                const double dep = P.ep_sqrt[ident[i]][ident[j]];
+                (t.chain[i] == t.chain[j] ? ehh_ii : ehh_ij) += dep;
+            }
+        }
+    for (size_t c = 0; c < t.chain_len.size(); ++c) {
+        const int s = t.chain_start[c], L = t.chain_len[c];
+        for (int i = s + L + 4; i < s + 2 * L; ++i)
+            if (bp[i] - 1 == i + L - 4) ++hb_alpha;
+    }
+    const double eps_hb = P.ep_sqrt[5][8];
+    const int sum_hb = hb_ii + hb_ij;
+    const double sumeps = -(sum_hb * eps_hb + (ehh_ii + ehh_ij));
+    double sumvel = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double m = ctx->raw.bmass[t.ident0[i]];
+        sumvel = sumvel + m * (sv[6 * i + 3] * sv[6 * i + 3] + sv[6 * i + 4] * sv[6 * i + 4] + sv[6 * i + 5] * sv[6 * i + 5]);
+    }
+    o->ered = 0.5 * sumvel + sumeps;
+    o->tred = sumvel / 3.0 / (double)N;
+    o->sumvel = sumvel;
+    o->hb_alpha = hb_alpha;
+    o->hb_ii = hb_ii;
+    o->hb_ij = hb_ij;
+    o->ehh_ii = ehh_ii;
+    o->ehh_ij = ehh_ij;
+    // radgyr.f:18-135 (backbone beads, chains unwrapped along the backbone)
+    double rg = 0.0, e2e = 0.0;
+    int nbb = 0;
+    for (size_t c = 0; c < t.chain_len.size(); ++c) {
+        const int s = t.chain_start[c], L = t.chain_len[c];
+        std::vector<double> q(3 * 4 * (size_t)L);
+        for (int k = 0; k < 4 * L; ++k)
+            for (int d3 = 0; d3 < 3; ++d3) q[3 * k + d3] = pos(s + k, d3);
+        auto unwrap = [&](int a, int b) {
+            for (int d3 = 0; d3 < 3; ++d3) {
+                const double dd = q[3 * a + d3] - q[3 * b + d3];
+                if (dd > 0.5) q[3 * a + d3] = q[3 * a + d3] - 1.0;
+                else if (dd < -1.0 * 0.5) q[3 * a + d3] = q[3 * a + d3] + 1.0;
+            }
+        };
+        for (int k = 1; k < L; ++k) unwrap(k, k - 1);
+        for (int k = 0; k < L; ++k) unwrap(L + k, k);
+        for (int k = 0; k < L; ++k) unwrap(2 * L + k, k);
+        double cx = 0, cy = 0, cz = 0, rgs = 0;
+        for (int k = 0; k < 3 * L; ++k) {
+            cx = cx + q[3 * k];
+            cy = cy + q[3 * k + 1];
+            cz = cz + q[3 * k + 2];
+        }
+        cx = cx / (double)(3 * L);
+        cy = cy / (double)(3 * L);
+        cz = cz / (double)(3 * L);
+        for (int k = 0; k < 3 * L; ++k) {
+            const double rx = (q[3 * k] - cx) * (q[3 * k] - cx), ry = (q[3 * k + 1] - cy) * (q[3 * k + 1] - cy),
+                         rz = (q[3 * k + 2] - cz) * (q[3 * k + 2] - cz);
+            rgs = rgs + rx + ry + rz;
+        }
+        rg = rg + rgs;
+        nbb += 3 * L;
+        // end2end.f: N of residue 1 to C of residue L
+        double e[3];
+        const int a = s + L, b = s + 3 * L - 1;
+        for (int d3 = 0; d3 < 3; ++d3) {
+            const double v = sv[6 * a + 3 + d3] - sv[6 * b + 3 + d3];
+            const double x = sv[6 * a + d3] - sv[6 * b + d3] + v * tf;
+            e[d3] = x - std::round(x);
+        }
+        e2e = e2e + std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    }
+    o->rg_avg = std::sqrt(rg / (double)nbb) * P.boxl_orig;
+    o->e2e_avg = e2e / (double)t.chain_len.size() * P.boxl_orig;
+    return 0;
+}
+
+int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_len, double density, double tstar,
+                  uint64_t seed, int32_t* restype, int32_t* hp, double* sv, double* boxl) {
+    if (!param_dir || nchains <= 0 || !chain_len || !restype || !hp || !sv || !boxl)
+        return fail("pdmd_generate: bad arguments");
+    Params raw;
+    std::string err;
+    if (!load_params(raw, param_dir, err)) return fail(err);
+    GenSpec g;
+    g.chain_len.assign(chain_len, chain_len + nchains);
+    g.density = density;
+    g.tstar = tstar;
+    g.seed = seed;
+    GenOut o;
+    if (!generate(raw, g, o, err)) return fail(err);
+    std::memcpy(restype, o.restype.data(), sizeof(int) * o.restype.size());
+    std::memcpy(hp, o.hp.data(), sizeof(int) * o.hp.size());
+    std::memcpy(sv, o.sv.data(), sizeof(double) * o.sv.size());
+    *boxl = o.boxl;
+    return 0;
+}
+
+}  // extern "C"

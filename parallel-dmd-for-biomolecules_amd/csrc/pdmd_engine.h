@@ -1,0 +1,21 @@
+// pdmd_engine.h -- launch wrappers of the gfx950 DMD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pdmd_device.h"
+
+namespace pdmd {
+
+// LDS bytes of one event-loop wavefront for a calendar of nblk 16-entry
+// blocks and a down-neighbour work list of maxd entries.
+size_t engine_lds_bytes(int nblk, int maxd);
+// Lists, pseudo-event seeds and events() for n systems (one wavefront each).
+hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, hipStream_t s);
+// Event loop: every listed system runs until its SysState::target.
+hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxd,
+                      hipStream_t s);
+// Test hook: collision times of (i, j) pairs (1-based) of system id.
+hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
+                          hipStream_t s);
+
+}  // namespace pdmd

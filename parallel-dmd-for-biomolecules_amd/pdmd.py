@@ -1,0 +1,252 @@
+"""Python host binding of the MI355X PRIME20 DMD engine (ctypes over include/pdmd.h).
+
+Mirrors the reference driver's path (parallel-dmd-PRIME20/code/main.F90): a
+system is set up from the reference's state arrays and parameter tables,
+then events are committed on the GPU in the reference's serial order.  The
+compute runs in libpdmd.so (gfx950 HIP); this module only moves arguments.
+Loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+PARAM_DIR = os.path.join(PKG_DIR, "parameters")
+LIB_PATH = os.path.join(PKG_DIR, "libpdmd.so")
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class SystemDesc(ctypes.Structure):
+    _fields_ = [
+        ("nchains", ctypes.c_int32),
+        ("chain_len", _i32p),
+        ("species", _i32p),
+        ("restype", _i32p),
+        ("hp", _i32p),
+        ("boxl", ctypes.c_double),
+        ("tstar", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("canon", ctypes.c_int32),
+        ("list_capacity", ctypes.c_int32),
+        ("sv", _f64p),
+        ("bptnr", _i32p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("coll", ctypes.c_int64),
+        ("pair_events", ctypes.c_int64),
+        ("numghosts", ctypes.c_int64),
+        ("nupdates", ctypes.c_int64),
+        ("nforced", ctypes.c_int64),
+        ("nevents", ctypes.c_int64 * 31),
+        ("t", ctypes.c_double),
+        ("tfalse", ctypes.c_double),
+        ("interval", ctypes.c_double),
+        ("interval_max", ctypes.c_double),
+        ("status", ctypes.c_int32),
+        ("max_list", ctypes.c_int32),
+        ("nbeads", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class Event(ctypes.Structure):
+    _fields_ = [
+        ("coll", ctypes.c_int64),
+        ("i", ctypes.c_int32),
+        ("j", ctypes.c_int32),
+        ("type", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("t", ctypes.c_double),
+    ]
+
+
+EVENT_DTYPE = np.dtype([("coll", "<i8"), ("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("pad", "<i4"), ("t", "<f8")])
+
+
+class Obs(ctypes.Structure):
+    _fields_ = [
+        ("ered", ctypes.c_double),
+        ("tred", ctypes.c_double),
+        ("sumvel", ctypes.c_double),
+        ("hb_alpha", ctypes.c_int32),
+        ("hb_ii", ctypes.c_int32),
+        ("hb_ij", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("ehh_ii", ctypes.c_double),
+        ("ehh_ij", ctypes.c_double),
+        ("rg_avg", ctypes.c_double),
+        ("e2e_avg", ctypes.c_double),
+    ]
+
+
+EXPORTS = [
+    "pdmd_version", "pdmd_last_error", "pdmd_ctx_create", "pdmd_ctx_destroy", "pdmd_system_add",
+    "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
+    "pdmd_download", "pdmd_stats", "pdmd_log_enable", "pdmd_log_read", "pdmd_predict_pairs", "pdmd_observables",
+    "pdmd_generate",
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded libpdmd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libpdmd.so not built: run make -C {PKG_DIR} (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pdmd_version.restype = ctypes.c_char_p
+        L.pdmd_last_error.restype = ctypes.c_char_p
+        L.pdmd_system_bytes.restype = ctypes.c_int64
+        L.pdmd_ctx_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.pdmd_ctx_destroy.argtypes = [ctypes.c_void_p]
+        L.pdmd_system_add.argtypes = [ctypes.c_void_p, ctypes.POINTER(SystemDesc), _i32p]
+        L.pdmd_system_remove.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.pdmd_nbeads.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.pdmd_system_bytes.argtypes = [ctypes.POINTER(SystemDesc)]
+        L.pdmd_advance.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
+        L.pdmd_advance_async.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
+        L.pdmd_wait.argtypes = [ctypes.c_void_p]
+        L.pdmd_download.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
+        L.pdmd_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]
+        L.pdmd_log_enable.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        L.pdmd_log_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, _i32p]
+        L.pdmd_predict_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f64p, _i32p]
+        L.pdmd_observables.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
+        L.pdmd_generate.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+class PdmdError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise PdmdError(f"pdmd error {rc}: {lib().pdmd_last_error().decode()}")
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype)) if a is not None else None
+
+
+def generate(chain_len, density=0.01, tstar=0.20, seed=1, param_dir=PARAM_DIR):
+    """Seeded synthetic PRIME20 system (sequences, coils, MB velocities)."""
+    cl = np.ascontiguousarray(chain_len, dtype=np.int32)
+    nres = int(cl.sum())
+    N = 4 * nres
+    restype = np.zeros(nres, np.int32)
+    hp = np.zeros(nres, np.int32)
+    sv = np.zeros(6 * N, np.float64)
+    boxl = ctypes.c_double(0.0)
+    _check(lib().pdmd_generate(param_dir.encode(), len(cl), _p(cl, ctypes.c_int32), density, tstar, seed,
+                               _p(restype, ctypes.c_int32), _p(hp, ctypes.c_int32), _p(sv, ctypes.c_double),
+                               ctypes.byref(boxl)))
+    return {"chain_len": cl, "restype": restype, "hp": hp, "sv": sv, "boxl": boxl.value, "tstar": tstar,
+            "density": density}
+
+
+class Engine:
+    """A device context holding resident systems (pdmd_ctx)."""
+
+    def __init__(self, param_dir=PARAM_DIR):
+        self._ctx = ctypes.c_void_p()
+        _check(lib().pdmd_ctx_create(param_dir.encode(), ctypes.byref(self._ctx)))
+        self._keep = {}
+        self.nbeads = {}
+
+    def close(self):
+        if self._ctx:
+            lib().pdmd_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def make_desc(system, seed, canon=True, bptnr=None, species=None, list_capacity=0):
+        keep = dict(cl=np.ascontiguousarray(system["chain_len"], np.int32),
+                    rt=np.ascontiguousarray(system["restype"], np.int32),
+                    hp=np.ascontiguousarray(system["hp"], np.int32),
+                    sv=np.ascontiguousarray(system["sv"], np.float64),
+                    bp=None if bptnr is None else np.ascontiguousarray(bptnr, np.int32),
+                    sp=None if species is None else np.ascontiguousarray(species, np.int32))
+        d = SystemDesc(len(keep["cl"]), _p(keep["cl"], ctypes.c_int32), _p(keep["sp"], ctypes.c_int32),
+                       _p(keep["rt"], ctypes.c_int32), _p(keep["hp"], ctypes.c_int32), float(system["boxl"]),
+                       float(system["tstar"]), int(seed), 1 if canon else 0, int(list_capacity),
+                       _p(keep["sv"], ctypes.c_double), _p(keep["bp"], ctypes.c_int32))
+        return d, keep
+
+    def add_system(self, system, seed, canon=True, bptnr=None, species=None, list_capacity=0):
+        d, keep = self.make_desc(system, seed, canon, bptnr, species, list_capacity)
+        sid = ctypes.c_int32(0)
+        _check(lib().pdmd_system_add(self._ctx, ctypes.byref(d), ctypes.byref(sid)))
+        self._keep[sid.value] = keep
+        self.nbeads[sid.value] = lib().pdmd_nbeads(self._ctx, sid.value)
+        return sid.value
+
+    def remove_system(self, sid):
+        _check(lib().pdmd_system_remove(self._ctx, sid))
+        self._keep.pop(sid, None)
+        self.nbeads.pop(sid, None)
+
+    def advance(self, ids, nevents, wait=True):
+        ids = np.ascontiguousarray(np.atleast_1d(ids), np.int32)
+        nev = np.ascontiguousarray(np.broadcast_to(np.asarray(nevents, np.int64), ids.shape), np.int64)
+        fn = lib().pdmd_advance if wait else lib().pdmd_advance_async
+        _check(fn(self._ctx, len(ids), _p(ids, ctypes.c_int32), _p(nev, ctypes.c_int64)))
+
+    def wait(self):
+        _check(lib().pdmd_wait(self._ctx))
+
+    def download(self, sid):
+        N = self.nbeads[sid]
+        out = dict(sv=np.zeros(6 * N), identity=np.zeros(N, np.int32), bptnr=np.zeros(N, np.int32),
+                   extra=np.zeros(4 * N, np.int32), tim=np.zeros(N + 3), nptnr=np.zeros(N + 3, np.int32),
+                   coltype=np.zeros(N + 3, np.int32))
+        _check(lib().pdmd_download(self._ctx, sid, _p(out["sv"], ctypes.c_double),
+                                   _p(out["identity"], ctypes.c_int32), _p(out["bptnr"], ctypes.c_int32),
+                                   _p(out["extra"], ctypes.c_int32), _p(out["tim"], ctypes.c_double),
+                                   _p(out["nptnr"], ctypes.c_int32), _p(out["coltype"], ctypes.c_int32)))
+        return out
+
+    def stats(self, sid):
+        s = Stats()
+        _check(lib().pdmd_stats(self._ctx, sid, ctypes.byref(s)))
+        return s
+
+    def log_enable(self, sid, capacity):
+        _check(lib().pdmd_log_enable(self._ctx, sid, capacity))
+
+    def log_read(self, sid, maxn):
+        out = np.zeros(maxn, EVENT_DTYPE)
+        n = ctypes.c_int32(0)
+        _check(lib().pdmd_log_read(self._ctx, sid, maxn, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n)))
+        return out[: n.value]
+
+    def predict_pairs(self, sid, i, j):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        tij = np.zeros(len(i))
+        ty = np.zeros(len(i), np.int32)
+        _check(lib().pdmd_predict_pairs(self._ctx, sid, len(i), _p(i, ctypes.c_int32), _p(j, ctypes.c_int32),
+                                        _p(tij, ctypes.c_double), _p(ty, ctypes.c_int32)))
+        return tij, ty
+
+    def observables(self, sid):
+        o = Obs()
+        _check(lib().pdmd_observables(self._ctx, sid, ctypes.byref(o)))
+        return o
