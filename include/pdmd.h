@@ -53,6 +53,7 @@ typedef struct {
     double t, tfalse, interval, interval_max;
     int32_t status, max_list;
     int32_t nbeads, pad;
+    double avg_up, avg_dn; /* mean up/down neighbour-list lengths (K_up, K_dn) */
 } pdmd_stats_t;
 
 typedef struct {
@@ -101,6 +102,9 @@ int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int6
  * blocks until the launch has finished. */
 int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* nevents);
 int pdmd_wait(pdmd_ctx* ctx);
+/* Device time of the most recent pdmd_advance* launch (HIP events on the
+ * context's stream), in milliseconds. */
+int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms);
 
 /* State in the reference layouts (any pointer may be NULL): sv(6,N),
  * identity(N), bptnr(N), extra_repuls(N,4) as extra[4*(k-1)+m-1],

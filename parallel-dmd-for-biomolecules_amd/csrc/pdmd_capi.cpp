@@ -76,6 +76,8 @@ struct pdmd_ctx {
     pdmd::DevBuf<int> d_ids;
     pdmd::DevBuf<long long> d_inc;
     hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
     int device = 0;
 };
 
@@ -169,6 +171,8 @@ int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device", -3);
     HIPCHK(hipGetDevice(&ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ctx->ev0));
+    HIPCHK(hipEventCreate(&ctx->ev1));
     *out = ctx.release();
     return 0;
 }
@@ -179,6 +183,8 @@ int pdmd_ctx_destroy(pdmd_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     delete ctx;
     return 0;
 }
@@ -498,7 +504,10 @@ int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const in
     }
     HIPCHK(hipMemcpyAsync(ctx->d_ids.p, dids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_inc.p, inc.data(), sizeof(long long) * nsys, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, 2 * max_cap + 8, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
     // the copies above read pageable memory synchronously before returning
     return 0;
 }
@@ -520,6 +529,14 @@ int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t*
         HIPCHK(hipMemcpy(&status, &H->st.p->status, sizeof(int), hipMemcpyDeviceToHost));
         if (status) return fail("system " + std::to_string(ids[k]) + " stopped with device status " + std::to_string(status), -10 - status);
     }
+    return 0;
+}
+
+int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms) {
+    if (!ctx || !ms) return fail("pdmd_last_launch_ms: null argument");
+    if (!ctx->timed) return fail("pdmd_last_launch_ms: no launch yet");
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    HIPCHK(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
     return 0;
 }
 
@@ -590,6 +607,16 @@ int pdmd_stats(pdmd_ctx* ctx, int32_t id, pdmd_stats_t* out) {
     out->status = st.status;
     out->max_list = (int32_t)st.max_nb_seen;
     out->nbeads = H->N;
+    std::vector<int> nu(H->N), nd(H->N);
+    HIPCHK(hipMemcpy(nu.data(), H->nup.p, sizeof(int) * H->N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nd.data(), H->ndn.p, sizeof(int) * H->N, hipMemcpyDeviceToHost));
+    long long su = 0, sd = 0;
+    for (int k = 0; k < H->N; ++k) {
+        su += nu[k];
+        sd += nd[k];
+    }
+    out->avg_up = (double)su / H->N;
+    out->avg_dn = (double)sd / H->N;
     return 0;
 }
 

@@ -53,6 +53,8 @@ class Stats(ctypes.Structure):
         ("max_list", ctypes.c_int32),
         ("nbeads", ctypes.c_int32),
         ("pad", ctypes.c_int32),
+        ("avg_up", ctypes.c_double),
+        ("avg_dn", ctypes.c_double),
     ]
 
 
@@ -90,7 +92,7 @@ EXPORTS = [
     "pdmd_version", "pdmd_last_error", "pdmd_ctx_create", "pdmd_ctx_destroy", "pdmd_system_add",
     "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
     "pdmd_download", "pdmd_stats", "pdmd_log_enable", "pdmd_log_read", "pdmd_predict_pairs", "pdmd_observables",
-    "pdmd_generate",
+    "pdmd_generate", "pdmd_last_launch_ms",
 ]
 
 _lib = None
@@ -115,6 +117,7 @@ def lib():
         L.pdmd_advance.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
         L.pdmd_advance_async.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
         L.pdmd_wait.argtypes = [ctypes.c_void_p]
+        L.pdmd_last_launch_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.pdmd_download.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
         L.pdmd_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]
         L.pdmd_log_enable.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
@@ -211,6 +214,11 @@ class Engine:
 
     def wait(self):
         _check(lib().pdmd_wait(self._ctx))
+
+    def last_launch_ms(self):
+        ms = ctypes.c_float(0.0)
+        _check(lib().pdmd_last_launch_ms(self._ctx, ctypes.byref(ms)))
+        return ms.value
 
     def download(self, sid):
         N = self.nbeads[sid]
