@@ -40,7 +40,7 @@ typedef struct {
     double tstar;
     uint64_t seed;
     int32_t canon;
-    int32_t list_capacity; /* per-bead up/down list slots; 0 = default (96) */
+    int32_t list_capacity; /* per-bead up/down list slots, at most 64; 0 = default (64) */
     const double* sv;
     const int32_t* bptnr;
 } pdmd_system_desc;

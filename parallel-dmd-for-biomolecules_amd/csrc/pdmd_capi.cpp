@@ -48,16 +48,16 @@ struct HostSys {
     Topology topo;
     Params p;  // scaled to this box
     Cutoffs cut{};
-    int N = 0, cap = 96, canon = 1;
+    int N = 0, cap = 64, canon = 1;
     double boxl = 0, tstar = 0;
     uint64_t seed = 0;
     // device memory
     DevBuf<DevTables> tab;
-    DevBuf<Bead> bead;
-    DevBuf<int> ident, bptnr, chain, loc, clen, res, restype, hp, nb_up, nup, nb_dn, ndn, nptnr, coltype, cbin,
-        cfine, ckey, cstart, crun, csorted;
+    DevBuf<Rec> rec;
+    DevBuf<double2> bw;
+    DevBuf<int> nb_up, nup, nb_dn, ndn, cfine, ckey, cstart, crun, csorted;
     DevBuf<int4> extra, ovr;
-    DevBuf<double> bondw, tim;
+    DevBuf<Leaf> leaf;
     DevBuf<double4> old_r;
     DevBuf<SysState> st;
     DevBuf<LogRec> log;
@@ -78,6 +78,7 @@ struct pdmd_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    size_t lds_limit = 65536;
     int device = 0;
 };
 
@@ -173,6 +174,20 @@ int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ctx->ev0));
     HIPCHK(hipEventCreate(&ctx->ev1));
+    // LDS per workgroup: opt in to the device maximum (160 KB on gfx950)
+    {
+        int lds = 0, optin = 0;
+        (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+        (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device);
+        size_t want = (size_t)std::max(lds, optin);
+        if (want < 65536) want = 65536;
+        if (engine_prepare(want) == hipSuccess)
+            ctx->lds_limit = want;
+        else {
+            (void)hipGetLastError();
+            ctx->lds_limit = (size_t)std::max(lds, 32768);
+        }
+    }
     *out = ctx.release();
     return 0;
 }
@@ -196,8 +211,8 @@ int64_t pdmd_system_bytes(const pdmd_system_desc* d) {
         N += 4 * d->chain_len[c];
         nres += d->chain_len[c];
     }
-    const int64_t cap = d->list_capacity > 0 ? d->list_capacity : 96;
-    int64_t b = N * (int64_t)sizeof(Bead) + N * 4 * 16 /*ints*/ + 2 * N * 16 /*extra, ovr*/ + N * 32 /*old_r*/ +
+    const int64_t cap = d->list_capacity > 0 ? d->list_capacity : 64;
+    int64_t b = N * (int64_t)sizeof(Rec) + N * 16 /*bw*/ + 2 * N * 16 /*extra, ovr*/ + N * 32 /*old_r*/ +
                 2 * N * cap * 4 + 2 * N * 4 + (N + 3) * 20 + nres * 48 + (int64_t)sizeof(DevTables);
     // coarse grid upper bound: box/rl cells per axis at the densest tables
     b += 4 * N * 4 + 2 * 4 * 1024 * 1024;
@@ -221,9 +236,12 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     const Topology& t = H.topo;
     const int N = t.N;
     H.N = N;
-    H.cap = d->list_capacity > 0 ? d->list_capacity : 96;
-    if (H.cap > 255) return fail("pdmd_system_add: list_capacity above 255");
-    if (N >= (1 << 24)) return fail("pdmd_system_add: too many beads");
+    H.cap = d->list_capacity > 0 ? d->list_capacity : 64;
+    if (H.cap > 64) return fail("pdmd_system_add: list_capacity above 64 (one wavefront lane per list slot)");
+    if (N > 65533) return fail("pdmd_system_add: at most 65533 beads per system");
+    if (nc > 8191) return fail("pdmd_system_add: at most 8191 chains per system");
+    for (int c = 0; c < nc; ++c)
+        if (clen[c] > 128) return fail("pdmd_system_add: at most 128 residues per chain");
     H.canon = d->canon ? 1 : 0;
     H.boxl = d->boxl;
     H.tstar = d->tstar;
@@ -232,9 +250,9 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     scale_params(H.p, d->boxl);
     const Params& P = H.p;
     // state: positions wrapped as inputinfo.f:90-95 and main.F90:210-216 do
-    std::vector<Bead> bead(N);
+    std::vector<Rec> bead(N);
     for (int k = 0; k < N; ++k) {
-        Bead b{};
+        Rec b{};
         double x = d->sv[6 * k], y = d->sv[6 * k + 1], z = d->sv[6 * k + 2];
         x = x - std::round(x);
         y = y - std::round(y);
@@ -324,22 +342,23 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     std::memcpy(tab.rlsq, H.cut.rlsq, sizeof tab.rlsq);
     for (int k = 0; k < 5; ++k) tab.shder[k] = P.shder_dist[k];
     tab.epsilon1 = P.epsilon[1];
-    // bond windows per residue (inputinfo.f:300-349, bond.f:21-31)
-    std::vector<double> bondw(6 * (size_t)nres);
-    for (int q = 0; q < nres; ++q) {
-        const int iii = rt[q] - 8;
-        const double bd = ctx->raw.drca[iii] / P.boxl_orig, brn = ctx->raw.drnh[iii] / P.boxl_orig,
-                     brc = ctx->raw.drco[iii] / P.boxl_orig;
-        const double dd = ctx->raw.del_rca[iii], dn = ctx->raw.del_rnh[iii], dc = ctx->raw.del_rco[iii];
-        bondw[6 * q + 0] = (1.0 - dd) * bd;
-        bondw[6 * q + 1] = (1.0 + dd) * bd;
-        bondw[6 * q + 2] = (1.0 - dn) * brn;
-        bondw[6 * q + 3] = (1.0 + dn) * brn;
-        bondw[6 * q + 4] = (1.0 - dc) * brc;
-        bondw[6 * q + 5] = (1.0 + dc) * brc;
+    // per-bead topology/identity words and own side-chain bond windows
+    // (inputinfo.f:300-349, bond.f:21-31: Calpha owns code 10, N code 11, C code 12)
+    std::vector<double2> bw(N, make_double2(0.0, 0.0));
+    for (int k = 0; k < N; ++k) {
+        const int c = t.chain[k], L = t.chain_len[c], kind = (t.loc[k] - 1) / L, r = (t.loc[k] - 1) % L + 1;
+        const int q = t.res[k];
+        const int pro = rt[q] == 17 ? 1 : 0;
+        bead[k].topo = make_topo(c, kind, r, L, hp[q] ? 1 : 0, pro);
+        bead[k].dyn = make_dyn(ident[k], bptnr[k]);
+        if (kind <= 2) {
+            const int iii = rt[q] - 8;
+            const double bl = (kind == 0 ? ctx->raw.drca[iii] : kind == 1 ? ctx->raw.drnh[iii] : ctx->raw.drco[iii]) /
+                              P.boxl_orig;
+            const double dl = kind == 0 ? ctx->raw.del_rca[iii] : kind == 1 ? ctx->raw.del_rnh[iii] : ctx->raw.del_rco[iii];
+            bw[k] = make_double2((1.0 - dl) * bl, (1.0 + dl) * bl);
+        }
     }
-    std::vector<int> clen_b(N);
-    for (int k = 0; k < N; ++k) clen_b[k] = t.chain_len[t.chain[k]];
     SysState st{};
     const double setemp = d->tstar * 12.0;  // main.F90:126-127
     st.t_fact = 0.00005;
@@ -348,32 +367,20 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     st.interval_max = st.n_forced * st.interval;
     st.sortsize = st.interval_max / (double)kNumbin;
     st.nbin = 1;
-    // uploads
     {
         std::vector<DevTables> tv(1, tab);
         HIPCHK(upload(H.tab, tv));
     }
-    HIPCHK(upload(H.bead, bead));
-    HIPCHK(upload(H.ident, ident));
-    HIPCHK(upload(H.bptnr, bptnr));
+    HIPCHK(upload(H.rec, bead));
+    HIPCHK(upload(H.bw, bw));
     HIPCHK(upload(H.extra, extra));
     HIPCHK(upload(H.ovr, ovr));
-    HIPCHK(upload(H.chain, t.chain));
-    HIPCHK(upload(H.loc, t.loc));
-    HIPCHK(upload(H.clen, clen_b));
-    HIPCHK(upload(H.res, t.res));
-    HIPCHK(upload(H.restype, rt));
-    HIPCHK(upload(H.hp, hp));
-    HIPCHK(upload(H.bondw, bondw));
     HIPCHK(H.old_r.alloc(N));
     HIPCHK(H.nb_up.alloc((size_t)N * H.cap));
     HIPCHK(H.nb_dn.alloc((size_t)N * H.cap));
     HIPCHK(H.nup.alloc(N));
     HIPCHK(H.ndn.alloc(N));
-    HIPCHK(H.tim.alloc(N + 3));
-    HIPCHK(H.nptnr.alloc(N + 3));
-    HIPCHK(H.coltype.alloc(N + 3));
-    HIPCHK(H.cbin.alloc(N + 3));
+    HIPCHK(H.leaf.alloc(N + 3));
     HIPCHK(H.cfine.alloc(N));
     HIPCHK(H.ckey.alloc(N));
     HIPCHK(H.csorted.alloc(N));
@@ -399,27 +406,16 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.hdelr = H.cut.hdelr;
     D.seed = d->seed;
     D.tab = H.tab.p;
-    D.bead = H.bead.p;
-    D.ident = H.ident.p;
-    D.bptnr = H.bptnr.p;
+    D.rec = H.rec.p;
+    D.bw = H.bw.p;
     D.extra = H.extra.p;
     D.ovr = H.ovr.p;
-    D.chain = H.chain.p;
-    D.loc = H.loc.p;
-    D.clen = H.clen.p;
-    D.res = H.res.p;
-    D.restype = H.restype.p;
-    D.hp = H.hp.p;
-    D.bondw = H.bondw.p;
     D.old_r = H.old_r.p;
     D.nb_up = H.nb_up.p;
     D.nup = H.nup.p;
     D.nb_dn = H.nb_dn.p;
     D.ndn = H.ndn.p;
-    D.tim = H.tim.p;
-    D.nptnr = H.nptnr.p;
-    D.coltype = H.coltype.p;
-    D.cbin = H.cbin.p;
+    D.leaf = H.leaf.p;
     D.cfine = H.cfine.p;
     D.ckey = H.ckey.p;
     D.cstart = H.cstart.p;
@@ -453,7 +449,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     if (ctx->d_ids.n < 1) HIPCHK(ctx->d_ids.alloc(4096));
     if (ctx->d_inc.n < 1) HIPCHK(ctx->d_inc.alloc(4096));
     HIPCHK(hipMemcpy(ctx->d_ids.p, &id, sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, 2 * H.cap + 8, ctx->stream));
+    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, 4 * H.cap + 16, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     SysState chk;
     HIPCHK(hipMemcpy(&chk, H.st.p, sizeof chk, hipMemcpyDeviceToHost));
@@ -505,7 +501,7 @@ int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const in
     HIPCHK(hipMemcpyAsync(ctx->d_ids.p, dids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_inc.p, inc.data(), sizeof(long long) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, 2 * max_cap + 8, ctx->stream));
+    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, 4 * max_cap + 16, ctx->lds_limit, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     // the copies above read pageable memory synchronously before returning
@@ -546,10 +542,10 @@ int pdmd_download(pdmd_ctx* ctx, int32_t id, double* sv, int32_t* identity, int3
     if (!H) return -1;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const int N = H->N;
-    if (sv) {
-        std::vector<Bead> b(N);
-        HIPCHK(hipMemcpy(b.data(), H->bead.p, sizeof(Bead) * N, hipMemcpyDeviceToHost));
-        for (int k = 0; k < N; ++k) {
+    std::vector<Rec> b(N);
+    HIPCHK(hipMemcpy(b.data(), H->rec.p, sizeof(Rec) * N, hipMemcpyDeviceToHost));
+    for (int k = 0; k < N; ++k) {
+        if (sv) {
             sv[6 * k] = b[k].x;
             sv[6 * k + 1] = b[k].y;
             sv[6 * k + 2] = b[k].z;
@@ -557,11 +553,8 @@ int pdmd_download(pdmd_ctx* ctx, int32_t id, double* sv, int32_t* identity, int3
             sv[6 * k + 4] = b[k].vy;
             sv[6 * k + 5] = b[k].vz;
         }
-    }
-    if (identity) HIPCHK(hipMemcpy(identity, H->ident.p, sizeof(int) * N, hipMemcpyDeviceToHost));
-    if (bptnr) {
-        HIPCHK(hipMemcpy(bptnr, H->bptnr.p, sizeof(int) * N, hipMemcpyDeviceToHost));
-        for (int k = 0; k < N; ++k) bptnr[k] = bptnr[k] + 1;
+        if (identity) identity[k] = dyn_ident(b[k].dyn);
+        if (bptnr) bptnr[k] = dyn_bptnr(b[k].dyn) + 1;
     }
     if (extra_repuls) {
         std::vector<int4> e(N);
@@ -573,13 +566,15 @@ int pdmd_download(pdmd_ctx* ctx, int32_t id, double* sv, int32_t* identity, int3
             extra_repuls[4 * k + 3] = e[k].w + 1;
         }
     }
-    if (tim) HIPCHK(hipMemcpy(tim, H->tim.p, sizeof(double) * (N + 3), hipMemcpyDeviceToHost));
-    if (nptnr) {
-        HIPCHK(hipMemcpy(nptnr, H->nptnr.p, sizeof(int) * (N + 3), hipMemcpyDeviceToHost));
-        for (int k = 0; k < N + 3; ++k)
-            if (nptnr[k] >= 0) nptnr[k] = nptnr[k] + 1;
+    if (tim || nptnr || coltype) {
+        std::vector<Leaf> lf(N + 3);
+        HIPCHK(hipMemcpy(lf.data(), H->leaf.p, sizeof(Leaf) * (N + 3), hipMemcpyDeviceToHost));
+        for (int k = 0; k < N + 3; ++k) {
+            if (tim) tim[k] = lf[k].tim;
+            if (nptnr) nptnr[k] = lf[k].nptnr >= 0 ? lf[k].nptnr + 1 : lf[k].nptnr;
+            if (coltype) coltype[k] = leaf_ct(lf[k].meta);
+        }
     }
-    if (coltype) HIPCHK(hipMemcpy(coltype, H->coltype.p, sizeof(int) * (N + 3), hipMemcpyDeviceToHost));
     return 0;
 }
 

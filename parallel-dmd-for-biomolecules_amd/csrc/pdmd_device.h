@@ -2,12 +2,15 @@
 // the host set-up code and the gfx950 kernels).
 //
 // Layout in HBM, one allocation set per system, structure-of-arrays:
-//   bead[N]        64 B  {x,y,z,vx,vy,vz,mass,-}: one 64-byte segment per
-//                        gather, the dominant access of the predictor
-//   ident/bptnr    int32 per bead, extra/ovr int4 per bead
-//   nb_up/nb_dn    int32 [N*cap]: partner | static pair code << 24
-//   tim/nptnr/coltype/cbin over N+3 calendar entries (beads, then the
-//                        ghost, sync and output pseudo-events)
+//   rec[N]      64 B  {x,y,z,vx,vy,vz,mass, topo, dyn}: one 64-byte segment
+//                     per gather carries everything a collision-time
+//                     prediction needs from a bead
+//   bw[N]       16 B  bond window of the bead's own side-chain bond
+//                     (Calpha: code 10, N: code 11, C: code 12)
+//   ovr/extra   int4  pair-code overrides and H-bond shoulder partners
+//   nb_up/nb_dn int32 [N*cap]: partner | static pair code << 24
+//   leaf[N+3]   16 B  {tim, nptnr, coltype|bin} per calendar entry: beads,
+//                     then the ghost, sync and output pseudo-events
 // Indices are 0-based on the device; the C ABI converts to the reference's
 // 1-based numbering.
 #pragma once
@@ -15,9 +18,35 @@
 
 namespace pdmd {
 
-struct __attribute__((aligned(16))) Bead {
-    double x, y, z, vx, vy, vz, bm, pad;
+// topo word: chain (13 bits) | kind (2: 0 Calpha, 1 N, 2 C, 3 R) | residue-1
+// (7) | chain length-1 (7) | hp (1) | proline residue (1)
+// dyn word:  identity (5 bits) | (H-bond partner + 1) << 5
+struct __attribute__((aligned(16))) Rec {
+    double x, y, z, vx, vy, vz, bm;
+    int topo, dyn;
 };
+
+struct __attribute__((aligned(16))) Leaf {
+    double tim;
+    int nptnr;
+    int meta;  // (bin + 1) in the low 16 bits, coltype in the high 16 (signed)
+};
+
+__host__ __device__ inline int topo_chain(int t) { return t & 0x1FFF; }
+__host__ __device__ inline int topo_kind(int t) { return (t >> 13) & 3; }
+__host__ __device__ inline int topo_res(int t) { return ((t >> 15) & 127) + 1; }
+__host__ __device__ inline int topo_len(int t) { return ((t >> 22) & 127) + 1; }
+__host__ __device__ inline int topo_hp(int t) { return (t >> 29) & 1; }
+__host__ __device__ inline int topo_pro(int t) { return (t >> 30) & 1; }
+__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro) {
+    return (chain & 0x1FFF) | (kind << 13) | ((r - 1) << 15) | ((L - 1) << 22) | (hp << 29) | (pro << 30);
+}
+__host__ __device__ inline int dyn_ident(int d) { return d & 31; }
+__host__ __device__ inline int dyn_bptnr(int d) { return (d >> 5) - 1; }
+__host__ __device__ inline int make_dyn(int ident, int bptnr) { return ident | ((bptnr + 1) << 5); }
+__host__ __device__ inline int leaf_bin(int meta) { return (meta & 0xFFFF) - 1; }
+__host__ __device__ inline int leaf_ct(int meta) { return meta >> 16; }
+__host__ __device__ inline int make_meta(int bin, int ct) { return ((bin + 1) & 0xFFFF) | (ct << 16); }
 
 // Scaled parameter tables of one system (box units), from
 // scale_down.f / make_code.f / nbor_setup.f.
@@ -40,8 +69,8 @@ struct SysState {
     long long coll, numghosts, nupdates, nforced;
     unsigned long long rng_ctr[2];
     long long nevents[32];
-    int nbin, status, log_count, rebuild_pending;
-    long long target;     // stop once coll reaches this
+    int nbin, status, log_count, pad1;
+    long long target;
     long long max_nb_seen;
 };
 
@@ -53,31 +82,20 @@ struct LogRec {
 
 struct SysDev {
     int N, nres, cap, canon;
-    int M, Mq, ncoarse, log_cap;      // fine cells per dim (interior), coarse cells per dim
+    int M, Mq, ncoarse, log_cap;  // fine cells per axis (interior), coarse cells per axis
     double width, half, setemp, avegtime, hdelr;
     unsigned long long seed;
     const DevTables* tab;
-    Bead* bead;
-    int* ident;
-    int* bptnr;
+    Rec* rec;
+    double2* bw;
     int4* extra;
     int4* ovr;
-    const int* chain;
-    const int* loc;
-    const int* clen;
-    const int* res;
-    const int* restype;
-    const int* hp;
-    const double* bondw;  // per residue: blmin/blmax for codes 10, 11, 12
     double4* old_r;
     int* nb_up;
     int* nup;
     int* nb_dn;
     int* ndn;
-    double* tim;
-    int* nptnr;
-    int* coltype;
-    int* cbin;
+    Leaf* leaf;
     int* cfine;    // packed fine cell coordinates per bead
     int* ckey;     // coarse cell per bead
     int* cstart;   // [ncoarse+1]
@@ -87,7 +105,7 @@ struct SysDev {
     LogRec* log;
 };
 
-// status codes (also returned through the C ABI, negated)
+// status codes (returned through the C ABI)
 enum : int {
     kOk = 0,
     kErrListCapacity = 1,

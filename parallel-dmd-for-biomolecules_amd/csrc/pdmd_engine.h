@@ -8,12 +8,14 @@ namespace pdmd {
 
 // LDS bytes of one event-loop wavefront for a calendar of nblk 16-entry
 // blocks and a down-neighbour work list of maxd entries.
-size_t engine_lds_bytes(int nblk, int maxd);
+size_t engine_lds_bytes(int nblk, int maxw, int leaf_cap);
+// Opt the event-loop kernels in to lds_limit bytes of LDS per workgroup.
+hipError_t engine_prepare(size_t lds_limit);
 // Lists, pseudo-event seeds and events() for n systems (one wavefront each).
 hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, hipStream_t s);
 // Event loop: every listed system runs until its SysState::target.
-hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxd,
-                      hipStream_t s);
+hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
+                      size_t lds_limit, hipStream_t s);
 // Test hook: collision times of (i, j) pairs (1-based) of system id.
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s);

@@ -1,14 +1,17 @@
 // pdmd_engine.hip -- gfx950 kernels of the PRIME20 discontinuous-MD event
-// loop.  One wavefront (64 lanes) owns one system for the whole launch and
+// loop.  One wavefront (64 lanes) owns one system for a whole launch and
 // commits its events in the reference's serial order; the wave
-// parallelises everything inside an event:
-//   * collision-time prediction over the candidate lists of the beads an
-//     event touches (FP64, reference evaluation order, no contraction),
-//   * the calendar: entries are grouped in 16-entry blocks whose minima
-//     live in LDS; the next event is a wave min-reduction over the block
-//     table, touched blocks are rescanned four at a time,
+// parallelises the work inside each event:
+//   * every collision-time prediction an event triggers (full recomputes
+//     of the colliding pair, single-pair updates of their down neighbours)
+//     is issued as one batch of 64-byte gathers, one lane per pair, FP64 in
+//     the reference's evaluation order (no contraction),
+//   * the calendar is a three-level min tree (16-entry leaf blocks, 64-block
+//     superblocks) whose block/superblock minima carry partner and type, so
+//     the next event comes out of one LDS min-reduction; leaves live in LDS
+//     when the system fits, else in HBM,
 //   * neighbour-list rebuilds (coarse-cell counting sort + per-bead scan),
-//     displacement checks and free-flight advances at the sync event.
+//     free flight and displacement checks at the sync event.
 // Many systems in one launch run concurrently, one wavefront each.
 #include <hip/hip_runtime.h>
 
@@ -23,14 +26,16 @@ namespace {
 
 constexpr int kW = 64;
 constexpr int kBlk = 16;
+constexpr int kSup = 64;
 constexpr int kNumbin = 2000;
+constexpr int kMaxBlist = 256;
 constexpr double kLt = 1e-10;
 constexpr double kSm = 5e-12;
 
 // ------------------------------------------------------------------------
-// project RNG (counter-based splitmix64) and logarithm from basic ops
+// project RNG (counter-based splitmix64) and a logarithm from basic ops
 // ------------------------------------------------------------------------
-__device__ __host__ inline uint64_t sm64(uint64_t x) {
+__device__ inline uint64_t sm64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
@@ -40,13 +45,15 @@ __device__ inline double rng_u01(uint64_t seed, int stream, uint64_t ctr) {
     const uint64_t key = sm64(seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(stream + 1)));
     return (double)(sm64(key + ctr) >> 11) * (1.0 / 9007199254740992.0);
 }
+// log x = e ln2 + 2 atanh((m-1)/(m+1)), m in [1/sqrt2, sqrt2]; identical
+// bits to the CPU oracle's restatement
 __device__ double dlog(double x) {
-    uint64_t b = __double_as_longlong(x);
+    uint64_t b = (uint64_t)__double_as_longlong(x);
     int e = (int)((b >> 52) & 0x7ff);
     int adj = 0;
     if (e == 0) {
         x = x * 18014398509481984.0;
-        b = __double_as_longlong(x);
+        b = (uint64_t)__double_as_longlong(x);
         e = (int)((b >> 52) & 0x7ff);
         adj = -54;
     }
@@ -77,16 +84,18 @@ __device__ double dlog(double x) {
 }
 
 // ------------------------------------------------------------------------
-// topology: static pair codes (closed form of make_code.f) and overrides
+// pair codes: closed form of make_code.f from two topo words, overrides
 // ------------------------------------------------------------------------
-__device__ int static_code(const SysDev& S, int a, int b) {
-    const int La = S.clen[a], Lb = S.clen[b];
-    int ka = (S.loc[a] - 1) / La, ra = (S.loc[a] - 1) % La + 1;
-    int kb = (S.loc[b] - 1) / Lb, rb = (S.loc[b] - 1) % Lb + 1;
-    if (S.chain[a] == S.chain[b]) {
+__device__ inline int static_t(int ta, int tb) {
+    int ka = topo_kind(ta), ra = topo_res(ta), kb = topo_kind(tb), rb = topo_res(tb);
+    if (topo_chain(ta) == topo_chain(tb)) {
         if (ka > kb || (ka == kb && ra > rb)) {
-            int t = ka; ka = kb; kb = t;
-            t = ra; ra = rb; rb = t;
+            int t = ka;
+            ka = kb;
+            kb = t;
+            t = ra;
+            ra = rb;
+            rb = t;
         }
         const int d = rb - ra;
         switch (ka * 4 + kb) {
@@ -103,44 +112,39 @@ __device__ int static_code(const SysDev& S, int a, int b) {
             case 7: return d == 0 ? 11 : (d == -1 ? 23 : 1);
             case 10: return d == 1 ? 21 : 1;
             case 11: return d == 0 ? 12 : (d == 1 ? 22 : (d == 2 ? 26 : 1));
-            case 15: return (d >= 4 && S.hp[S.res[a]] == 1 && S.hp[S.res[b]] == 1) ? 16 : 1;
+            case 15: return (d >= 4 && topo_hp(ta) && topo_hp(tb)) ? 16 : 1;
             default: return 1;
         }
     }
-    if (ka == 3 && kb == 3) return (S.hp[S.res[a]] == 1 && S.hp[S.res[b]] == 1) ? 16 : 1;
-    if ((ka == 1 && kb == 2) || (ka == 2 && kb == 1)) {
-        const int nbd = ka == 1 ? a : b, cbd = ka == 1 ? b : a;
-        if (S.restype[S.res[nbd]] == 17 && S.chain[nbd] < S.chain[cbd]) return 1;
-        return 15;
-    }
+    if (ka == 3 && kb == 3) return (topo_hp(ta) && topo_hp(tb)) ? 16 : 1;
+    if (ka == 1 && kb == 2) return (topo_pro(ta) && topo_chain(ta) < topo_chain(tb)) ? 1 : 15;
+    if (ka == 2 && kb == 1) return (topo_pro(tb) && topo_chain(tb) < topo_chain(ta)) ? 1 : 15;
     return 1;
 }
-
-// ev_code(row, col): a dynamic override written by the H-bond shoulder
-// bookkeeping, else the static code.
-__device__ inline int dyn_code(const SysDev& S, int row, int col, int stat) {
-    const int4 o = S.ovr[row];
+// ev_code(row, col) with row's override slots o
+__device__ inline int dyn_code(int4 o, int col, int stat) {
     if (o.x >= 0 && (o.x >> 8) == col) return o.x & 0xff;
     if (o.y >= 0 && (o.y >> 8) == col) return o.y & 0xff;
     if (o.z >= 0 && (o.z >> 8) == col) return o.z & 0xff;
     if (o.w >= 0 && (o.w >> 8) == col) return o.w & 0xff;
     return stat;
 }
-// single-lane writer of ev_code(row, col) = code
+// single-lane writer of ev_code(row, col) = code (keeps an override only
+// where it differs from the static code)
 __device__ void set_code(const SysDev& S, int row, int col, int code, int& status) {
     int4 o = S.ovr[row];
-    int* s = &o.x;
-    const int stat = static_code(S, row, col);
-    for (int k = 0; k < 4; ++k)
-        if (s[k] >= 0 && (s[k] >> 8) == col) s[k] = -1;
+    const int stat = static_t(S.rec[row].topo, S.rec[col].topo);
+    if (o.x >= 0 && (o.x >> 8) == col) o.x = -1;
+    if (o.y >= 0 && (o.y >> 8) == col) o.y = -1;
+    if (o.z >= 0 && (o.z >> 8) == col) o.z = -1;
+    if (o.w >= 0 && (o.w >> 8) == col) o.w = -1;
     if (code != stat) {
-        int k = 0;
-        while (k < 4 && s[k] >= 0) ++k;
-        if (k == 4) {
-            status = kErrOverrideSlots;
-            return;
-        }
-        s[k] = (col << 8) | code;
+        const int v = (col << 8) | code;
+        if (o.x < 0) o.x = v;
+        else if (o.y < 0) o.y = v;
+        else if (o.z < 0) o.z = v;
+        else if (o.w < 0) o.w = v;
+        else status = kErrOverrideSlots;
     }
     S.ovr[row] = o;
 }
@@ -151,7 +155,7 @@ __device__ void set_code(const SysDev& S, int row, int col, int code, int& statu
 struct Geo {
     double rx, ry, rz, b, r2, v2;
 };
-__device__ inline void geo(const Bead& a, const Bead& c, double tf, Geo& g) {
+__device__ inline void geo(const Rec& a, const Rec& c, double tf, Geo& g) {
     const double vx = a.vx - c.vx, vy = a.vy - c.vy, vz = a.vz - c.vz;
     double rx = a.x - c.x + vx * tf;
     double ry = a.y - c.y + vy * tf;
@@ -166,31 +170,29 @@ __device__ inline void geo(const Bead& a, const Bead& c, double tf, Geo& g) {
     g.r2 = rx * rx + ry * ry + rz * rz;
     g.v2 = vx * vx + vy * vy + vz * vz;
 }
-
-__device__ inline void bond_window(const SysDev& S, const DevTables& T, int code, int res_i, double& blmin,
-                                   double& blmax) {
-    blmin = T.ev_param[2][code];
-    blmax = T.ev_param[3][code];
-    if (code >= 10 && code <= 12) {
-        const double* w = S.bondw + (size_t)res_i * 6 + (code - 10) * 2;
-        blmin = w[0];
-        blmax = w[1];
+__device__ inline void bond_window(const DevTables& T, int code, double2 bw, double& blmin, double& blmax) {
+    if (code >= 10) {
+        blmin = bw.x;
+        blmax = bw.y;
+    } else {
+        blmin = T.ev_param[2][code];
+        blmax = T.ev_param[3][code];
     }
 }
-
-// Collision time of pair (i, j) at false time tf with ev_code `code`; tij
-// keeps its preset (1e9) and type its value when no event is predicted.
-__device__ void predict(const SysDev& S, const DevTables& T, int code, const Bead& bi, const Bead& bj, int idi,
-                        int idj, bool bonded_ij, int res_i, double tf, double& tij, int& type) {
+// collision time of pair (a, c) with ev_code `code` at false time tf; tij
+// keeps its preset (1e9) and type its value when no event is predicted
+__device__ void predict(const DevTables& T, int code, const Rec& a, const Rec& c, bool bonded, double2 bwa,
+                        double tf, double& tij, int& type) {
+    const int ia = dyn_ident(a.dyn), ic = dyn_ident(c.dyn);
     Geo g;
-    geo(bi, bj, tf, g);
+    geo(a, c, tf, g);
     const double b = g.b, r2 = g.r2, v2 = g.v2;
     if (code <= 3 || (code >= 17 && code <= 26)) {  // core.f:14-40
         if (b < 0.0) {
             const double pe = T.ev_param[1][code];
-            double sigsq = T.sigma_sq[idi][idj] * (pe * pe);
-            if (code >= 22 && code <= 26) {
-                const double q = T.sqz610[code - 21][idi > idj ? idi : idj];
+            double sigsq = T.sigma_sq[ia][ic] * (pe * pe);
+            if (code >= 22) {
+                const double q = T.sqz610[code - 21][ia > ic ? ia : ic];
                 sigsq = sigsq * (q * q);
             }
             const double discr = b * b - v2 * (r2 - sigsq);
@@ -201,7 +203,7 @@ __device__ void predict(const SysDev& S, const DevTables& T, int code, const Bea
         }
     } else if (code <= 12) {  // bond.f:17-117
         double blmin, blmax;
-        bond_window(S, T, code, res_i, blmin, blmax);
+        bond_window(T, code, bwa, blmin, blmax);
         if (b < 0.0) {
             const double d1 = b * b - v2 * (r2 - blmin * blmin);
             if (d1 > 0.0) {
@@ -222,11 +224,11 @@ __device__ void predict(const SysDev& S, const DevTables& T, int code, const Bea
             }
         }
     } else if (code == 15) {  // nc_sqwel.f:19-122
-        const double diff = r2 - T.welldia_sq[idi][idj];
-        if (idi + idj == 5) {
+        const double diff = r2 - T.welldia_sq[ia][ic];
+        if (ia + ic == 5) {
             if (b < 0.0) {
                 if (diff < 0.0) {
-                    const double cd = b * b - v2 * (r2 - T.sigma_sq[idi][idj]);
+                    const double cd = b * b - v2 * (r2 - T.sigma_sq[ia][ic]);
                     if (cd > 0.0) {
                         tij = (-b - sqrt(cd)) / v2;
                         type = 1;
@@ -247,9 +249,9 @@ __device__ void predict(const SysDev& S, const DevTables& T, int code, const Bea
                 tij = (-b + sqrt(wd)) / v2;
                 type = 16;
             }
-        } else if (bonded_ij) {
+        } else if (bonded) {
             if (b < 0.0) {
-                const double fs = T.sigma_sq[idi][idj] * T.ev_param[1][15] * T.ev_param[1][15];
+                const double fs = T.sigma_sq[ia][ic] * T.ev_param[1][15] * T.ev_param[1][15];
                 const double fc = b * b - v2 * (r2 - fs);
                 if (fc > 0.0) {
                     tij = (-b - sqrt(fc)) / v2;
@@ -272,10 +274,10 @@ __device__ void predict(const SysDev& S, const DevTables& T, int code, const Bea
             }
         }
     } else if (code == 16) {  // sqwel.f:17-63
-        const double diff = r2 - T.welldia_sq[idi][idj];
+        const double diff = r2 - T.welldia_sq[ia][ic];
         if (b < 0.0) {
             if (diff < 0.0) {
-                const double cd = b * b - v2 * (r2 - T.sigma_sq[idi][idj]);
+                const double cd = b * b - v2 * (r2 - T.sigma_sq[ia][ic]);
                 if (cd > 0.0) {
                     tij = (-b - sqrt(cd)) / v2;
                     type = 1;
@@ -297,10 +299,10 @@ __device__ void predict(const SysDev& S, const DevTables& T, int code, const Bea
             type = 8;
         }
     } else if (code >= 40) {  // sqshlder.f:17-62
-        const double diff = r2 - T.shlddia_sq[idi][idj];
+        const double diff = r2 - T.shlddia_sq[ia][ic];
         if (b < 0.0) {
             if (diff < 0.0) {
-                const double cd = b * b - v2 * (r2 - T.sigma_sq[idi][idj]);
+                const double cd = b * b - v2 * (r2 - T.sigma_sq[ia][ic]);
                 if (cd > 0.0) {
                     tij = (-b - sqrt(cd)) / v2;
                     type = 1;
@@ -326,37 +328,38 @@ __device__ void predict(const SysDev& S, const DevTables& T, int code, const Bea
     }
 }
 
-// prediction of pair (x, m) with the reference's code lookup: up-pairs read
-// ev_code(m, x) (eventredo_up.f:28), down-pairs ev_code(x, m) (eventredo_down.f:28)
-__device__ inline void predict_pair(const SysDev& S, const DevTables& T, int x, int m, int code, double tf,
-                                    double& tij, int& type) {
-    const Bead bx = S.bead[x];
-    const Bead bm = S.bead[m];
-    tij = 1000000000.0;
-    type = 0;
-    predict(S, T, code, bx, bm, S.ident[x], S.ident[m], S.bptnr[x] == m, S.res[x], tf, tij, type);
-}
-
 // ------------------------------------------------------------------------
 // wave helpers
 // ------------------------------------------------------------------------
-__device__ inline bool key_less(int b1, double t1, int i1, int b2, double t2, int i2) {
-    if (b1 != b2) return b1 < b2;
-    if (t1 != t2) return t1 < t2;
-    return i1 < i2;
+__device__ inline Rec shfl_rec(const Rec& r, int src) {
+    Rec o;
+    o.x = __shfl(r.x, src, kW);
+    o.y = __shfl(r.y, src, kW);
+    o.z = __shfl(r.z, src, kW);
+    o.vx = __shfl(r.vx, src, kW);
+    o.vy = __shfl(r.vy, src, kW);
+    o.vz = __shfl(r.vz, src, kW);
+    o.bm = __shfl(r.bm, src, kW);
+    o.topo = __shfl(r.topo, src, kW);
+    o.dyn = __shfl(r.dyn, src, kW);
+    return o;
 }
-__device__ inline void wave_min_key(int& b, double& t, int& i, int width) {
-    for (int m = 1; m < width; m <<= 1) {
-        const int ob = __shfl_xor(b, m, kW);
-        const double ot = __shfl_xor(t, m, kW);
-        const int oi = __shfl_xor(i, m, kW);
-        if (key_less(ob, ot, oi, b, t, i)) {
-            b = ob;
-            t = ot;
-            i = oi;
-        }
+__device__ inline int wave_or(int v) {
+    for (int s = 1; s < kW; s <<= 1) v |= __shfl_xor(v, s, kW);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int s = 1; s < kW; s <<= 1) {
+        const double o = __shfl_xor(v, s, kW);
+        v = o > v ? o : v;
     }
+    return v;
 }
+__device__ inline int wave_imax(int v) {
+    for (int s = 1; s < kW; s <<= 1) v = max(v, __shfl_xor(v, s, kW));
+    return v;
+}
+// min of predictions (t, position) carrying partner and type
 __device__ inline void wave_min_pred(double& t, int& p, int& m, int& ty) {
     for (int s = 1; s < kW; s <<= 1) {
         const double ot = __shfl_xor(t, s, kW);
@@ -371,26 +374,39 @@ __device__ inline void wave_min_pred(double& t, int& p, int& m, int& ty) {
         }
     }
 }
-__device__ inline double wave_max(double v) {
-    for (int s = 1; s < kW; s <<= 1) {
-        const double o = __shfl_xor(v, s, kW);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-__device__ inline int wave_or(int v) {
-    for (int s = 1; s < kW; s <<= 1) v |= __shfl_xor(v, s, kW);
-    return v;
-}
-__device__ inline int wave_sum(int v) {
-    for (int s = 1; s < kW; s <<= 1) v += __shfl_xor(v, s, kW);
-    return v;
-}
-__device__ inline unsigned long long ballot(bool p) { return __ballot(p); }
 
-// ------------------------------------------------------------------------
+// calendar tree entries
+struct BEnt {
+    double t;
+    int bin, idx, p, ct;
+};
+__device__ inline bool key_less(int b1, double t1, int i1, int b2, double t2, int i2) {
+    if (b1 != b2) return b1 < b2;
+    if (t1 != t2) return t1 < t2;
+    return i1 < i2;
+}
+__device__ inline void red_key(BEnt& k, int width) {
+    for (int m = 1; m < width; m <<= 1) {
+        BEnt o;
+        o.t = __shfl_xor(k.t, m, kW);
+        o.bin = __shfl_xor(k.bin, m, kW);
+        o.idx = __shfl_xor(k.idx, m, kW);
+        o.p = __shfl_xor(k.p, m, kW);
+        o.ct = __shfl_xor(k.ct, m, kW);
+        if (key_less(o.bin, o.t, o.idx, k.bin, k.t, k.idx)) k = o;
+    }
+}
+__device__ inline BEnt empty_ent() {
+    BEnt e;
+    e.t = 0.0;
+    e.bin = INT_MAX;
+    e.idx = INT_MAX;
+    e.p = -1;
+    e.ct = 0;
+    return e;
+}
+
 // per-system register state
-// ------------------------------------------------------------------------
 struct Sc {
     double tfalse, t, interval, interval_max, t_fact, n_forced, sortsize, tbin_off, old_tfalse;
     long long coll, numghosts, nupdates, nforced, target;
@@ -406,151 +422,180 @@ __device__ inline int calc_bin(const Sc& c, double tim, bool uncond, int& status
         status = kErrBin;
         return -1;
     }
-    return j < c.nbin ? -1 : j;  // behind the scan position: never reached before the next rebuild
+    // inserted behind the scan position: the reference's bucket scan
+    // (main.F90:496-498) never returns to it before the next rebuild
+    return j < c.nbin ? -1 : j;
 }
 
-struct Lds {
-    double* bt_t;
-    int* bt_b;
-    int* bt_i;
-    int* dl_l;
-    int* dl_tag;
-    int* blist;
-    int* misc;
-    int nblk, maxd, maxblist;
+struct Pair {  // the event's colliding pair (j < 0 for a ghost event)
+    int i, j;
+    Rec ri, rj;
+    double2 bwi, bwj;
+    int4 xi, xj;
 };
 
-// rescan the listed calendar blocks, four 16-entry blocks per wave pass
-__device__ void rescan_blocks(const SysDev& S, const Lds& L, int N3, int nb) {
+// LDS views
+struct Lds {
+    DevTables* tab;
+    BEnt* blk;
+    BEnt* sup;
+    int* wm;     // work list: partner / down bead
+    int* wx;     // work list: owner / op tag
+    int* blist;  // calendar blocks to rescan
+    Pair* pair;  // the current event's pair (wave-uniform, kept out of registers)
+    Sc* sc;      // the system's scalars (wave-uniform, kept out of registers)
+    Leaf* leaf;  // calendar leaves, in LDS or HBM
+    int nblk, nsup, maxw;
+    bool leaf_lds;
+};
+
+__device__ void rescan_block_list(const Lds& L, int N3, int nb) {
     const int lane = threadIdx.x;
     for (int base = 0; base < nb; base += 4) {
         const int q = base + (lane >> 4), k = lane & 15;
-        const int blk = q < nb ? L.blist[q] : -1;
-        int kb = INT_MAX, ki = INT_MAX;
-        double kt = 0.0;
-        const int e = blk * kBlk + k;
-        if (blk >= 0 && e < N3) {
-            const int cb = S.cbin[e];
-            if (cb >= 0) {
-                kb = cb;
-                kt = S.tim[e];
-                ki = e;
+        const int b = q < nb ? L.blist[q] : -1;
+        BEnt e = empty_ent();
+        const int idx = b * kBlk + k;
+        if (b >= 0 && idx < N3) {
+            const Leaf lf = L.leaf[idx];
+            const int bin = leaf_bin(lf.meta);
+            if (bin >= 0) {
+                e.t = lf.tim;
+                e.bin = bin;
+                e.idx = idx;
+                e.p = lf.nptnr;
+                e.ct = leaf_ct(lf.meta);
             }
         }
-        wave_min_key(kb, kt, ki, 16);
-        if (k == 0 && blk >= 0) {
-            L.bt_t[blk] = kt;
-            L.bt_b[blk] = kb;
-            L.bt_i[blk] = ki;
-        }
+        red_key(e, 16);
+        if (k == 0 && b >= 0) L.blk[b] = e;
+    }
+    __syncthreads();
+    // superblocks of the touched blocks
+    for (int q = 0; q < nb; ++q) {
+        const int s = L.blist[q] / kSup;
+        bool seen = false;
+        for (int r = 0; r < q; ++r)
+            if (L.blist[r] / kSup == s) seen = true;
+        if (seen) continue;
+        const int b = s * kSup + lane;
+        BEnt e = b < L.nblk ? L.blk[b] : empty_ent();
+        red_key(e, kW);
+        if (lane == 0) L.sup[s] = e;
     }
     __syncthreads();
 }
-__device__ void rescan_all(const SysDev& S, const Lds& L, int N3) {
+__device__ void rescan_all(const Lds& L, int N3) {
     const int lane = threadIdx.x;
     for (int base = 0; base < L.nblk; base += 4) {
-        const int blk = base + (lane >> 4), k = lane & 15;
-        int kb = INT_MAX, ki = INT_MAX;
-        double kt = 0.0;
-        const int e = blk * kBlk + k;
-        if (blk < L.nblk && e < N3) {
-            const int cb = S.cbin[e];
-            if (cb >= 0) {
-                kb = cb;
-                kt = S.tim[e];
-                ki = e;
+        const int b = base + (lane >> 4), k = lane & 15;
+        BEnt e = empty_ent();
+        const int idx = b * kBlk + k;
+        if (b < L.nblk && idx < N3) {
+            const Leaf lf = L.leaf[idx];
+            const int bin = leaf_bin(lf.meta);
+            if (bin >= 0) {
+                e.t = lf.tim;
+                e.bin = bin;
+                e.idx = idx;
+                e.p = lf.nptnr;
+                e.ct = leaf_ct(lf.meta);
             }
         }
-        wave_min_key(kb, kt, ki, 16);
-        if (k == 0 && blk < L.nblk) {
-            L.bt_t[blk] = kt;
-            L.bt_b[blk] = kb;
-            L.bt_i[blk] = ki;
-        }
+        red_key(e, 16);
+        if (k == 0 && b < L.nblk) L.blk[b] = e;
+    }
+    __syncthreads();
+    for (int s = 0; s < L.nsup; ++s) {
+        const int b = s * kSup + lane;
+        BEnt e = b < L.nblk ? L.blk[b] : empty_ent();
+        red_key(e, kW);
+        if (lane == 0) L.sup[s] = e;
     }
     __syncthreads();
 }
 
-// Full recompute of the next event of up to two beads x1, x2 (x2 < 0: none)
-// over their up lists and upper shoulder partners: the loops of
-// partial_events.f:26-46 / events.f:24-104.  Results are wave-uniform.
-__device__ void wave_full2(const SysDev& S, const DevTables& T, int x1, int x2, double tf, double T0,
-                           double& tim1, int& np1, int& ct1, double& tim2, int& np2, int& ct2, int& status) {
-    const int lane = threadIdx.x;
-    const int cap = S.cap;
-    int nx[2], ex[2][3], nex[2];
-    const int xs[2] = {x1, x2};
-    for (int q = 0; q < 2; ++q) {
-        nx[q] = 0;
-        nex[q] = 0;
-        if (xs[q] < 0) continue;
-        nx[q] = S.nup[xs[q]];
-        const int4 e = S.extra[xs[q]];
-        const int ev[3] = {e.x, e.y, e.z};
-        for (int k = 0; k < 3; ++k)
-            if (ev[k] > xs[q]) ex[q][nex[q]++] = ev[k];
-    }
-    const int tot1 = nx[0] + nex[0], tot = tot1 + nx[1] + nex[1];
-    double bt[2] = {1e300, 1e300};
-    int bp[2] = {INT_MAX, INT_MAX}, bm[2] = {-1, -1}, bty[2] = {-1, -1};
+// ------------------------------------------------------------------------
+// rescheduling after an event: partial_events.f:16-201
+// ------------------------------------------------------------------------
+__device__ inline Rec load_rec(const SysDev& S, const Pair& P, int m) {
+    if (m == P.i) return P.ri;
+    if (m == P.j) return P.rj;
+    return S.rec[m];
+}
+
+// Cooperative full recompute of bead l (rl, bwl, xl wave-uniform): min over
+// l's up list and upper shoulder partners (partial_events.f:121-139).
+__device__ void full_one(const SysDev& S, const DevTables& T, const Pair& P, int l, const Rec& rl, double2 bwl,
+                         int4 xl, double tf, double T0, double& tim, int& np, int& ct, int& status) {
+    const int lane = threadIdx.x, cap = S.cap;
+    const int nu = S.nup[l];
+    const int nx = (xl.x > l ? 1 : 0) + (xl.y > l ? 1 : 0) + (xl.z > l ? 1 : 0);
+    const int tot = nu + nx;
+    double bt = 1e300;
+    int bp = INT_MAX, bm = -1, bty = -1;
     for (int base = 0; base < tot; base += kW) {
         const int p = base + lane;
         if (p < tot) {
-            const int q = p < tot1 ? 0 : 1;
-            const int pp = q == 0 ? p : p - tot1;
-            const int x = xs[q];
             int m, stat;
-            if (pp < nx[q]) {
-                const int ent = S.nb_up[(size_t)x * cap + pp];
+            if (p < nu) {
+                const int ent = S.nb_up[(size_t)l * cap + p];
                 m = ent & 0xFFFFFF;
                 stat = ent >> 24;
             } else {
-                m = ex[q][pp - nx[q]];
-                stat = static_code(S, m, x);
+                const int k = p - nu;
+                int cnt = 0;
+                m = -1;
+                if (xl.x > l) {
+                    if (cnt == k) m = xl.x;
+                    ++cnt;
+                }
+                if (xl.y > l) {
+                    if (cnt == k) m = xl.y;
+                    ++cnt;
+                }
+                if (xl.z > l) {
+                    if (cnt == k) m = xl.z;
+                    ++cnt;
+                }
+                stat = -1;
             }
-            const int code = dyn_code(S, m, x, stat);
-            double tij;
-            int ty;
-            predict_pair(S, T, x, m, code, tf, tij, ty);
+            const Rec rm = load_rec(S, P, m);
+            const int4 om = S.ovr[m];
+            if (stat < 0) stat = static_t(rm.topo, rl.topo);
+            const int code = dyn_code(om, l, stat);  // ev_code(m, l), eventredo_up.f:28
+            double tij = 1000000000.0;
+            int ty = 0;
+            predict(T, code, rl, rm, dyn_bptnr(rl.dyn) == m, bwl, tf, tij, ty);
             if (ty == -99) status = kErrBadCode;
             if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
-            if (tij < bt[q]) {  // lanes see their candidates in list order
-                bt[q] = tij;
-                bp[q] = p;
-                bm[q] = m;
-                bty[q] = ty;
+            if (tij < bt) {
+                bt = tij;
+                bp = p;
+                bm = m;
+                bty = ty;
             }
         }
     }
-    for (int q = 0; q < 2; ++q) wave_min_pred(bt[q], bp[q], bm[q], bty[q]);
-    // tim = min(T0, candidates) + tf, strict '<' keeps the first minimum
-    if (bt[0] < T0) {
-        tim1 = bt[0] + tf;
-        np1 = bm[0];
-        ct1 = bty[0];
+    wave_min_pred(bt, bp, bm, bty);
+    if (bt < T0) {
+        tim = bt + tf;
+        np = bm;
+        ct = bty;
     } else {
-        tim1 = T0 + tf;
-        np1 = -1;
-        ct1 = -1;
-    }
-    if (bt[1] < T0) {
-        tim2 = bt[1] + tf;
-        np2 = bm[1];
-        ct2 = bty[1];
-    } else {
-        tim2 = T0 + tf;
-        np2 = -1;
-        ct2 = -1;
+        tim = T0 + tf;
+        np = -1;
+        ct = -1;
     }
 }
 
-// eventredo_down.f: single pair update of l's next event with partner p
-__device__ inline void single_update(const SysDev& S, const DevTables& T, int l, int p, double tf, double& st_t,
-                                     int& st_p, int& st_c, bool& modified, int& status) {
-    const int code = dyn_code(S, l, p, static_code(S, l, p));
-    double tij;
-    int ty;
-    predict_pair(S, T, l, p, code, tf, tij, ty);
+// eventredo_down.f: single-pair update of (st_t, st_p, st_c) of bead l
+__device__ inline void single_update(const DevTables& T, const Rec& rl, int4 ol, double2 bwl, int p, const Rec& rp,
+                                     double tf, double& st_t, int& st_p, int& st_c, bool& modified, int& status) {
+    const int code = dyn_code(ol, p, static_t(rl.topo, rp.topo));  // ev_code(l, p)
+    double tij = 1000000000.0;
+    int ty = 0;
+    predict(T, code, rl, rp, dyn_bptnr(rl.dyn) == p, bwl, tf, tij, ty);
     if (ty == -99) status = kErrBadCode;
     if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
     tij = tij + tf;
@@ -562,134 +607,192 @@ __device__ inline void single_update(const SysDev& S, const DevTables& T, int l,
     }
 }
 
-// partial_events.f:16-201 for event (i, j); j < 0 for a ghost event.
-// Returns (through L.blist / *nblist) the calendar blocks to rescan.
-__device__ void partial_events(const SysDev& S, const DevTables& T, const Lds& L, Sc& c, int i, int j,
-                               bool xpulse_del) {
-    const int lane = threadIdx.x;
-    const int N3 = S.N + 3;
+// extra_blk: a further calendar block to rescan (the ghost entry), or -1
+__device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc& c, const Pair& P, int extra_blk,
+                           bool xpulse_del) {
+    const int lane = threadIdx.x, cap = S.cap, N3 = S.N + 3;
+    const int i = P.i, j = P.j;
     const double tf = c.tfalse;
     const double T0 = c.interval_max + kLt - tf;
     int status = 0;
-    // (a) full recompute of i and j
-    double ti, tj;
-    int pi, ci, pj, cj;
-    wave_full2(S, T, i, j, tf, T0, ti, pi, ci, tj, pj, cj, status);
-    int nbl = 0;
-    if (lane == 0) {
-        S.tim[i] = ti;
-        S.nptnr[i] = pi;
-        S.coltype[i] = ci;
-        S.cbin[i] = calc_bin(c, ti, false, status);
-        L.blist[nbl] = i / kBlk;
-        if (j >= 0) {
-            S.tim[j] = tj;
-            S.nptnr[j] = pj;
-            S.coltype[j] = cj;
-            S.cbin[j] = calc_bin(c, tj, false, status);
-            L.blist[nbl + 1] = j / kBlk;
-        }
+    // ---- list rows of i and j, one entry per lane (cap <= 64)
+    const int nui = S.nup[i], ndi = S.ndn[i];
+    const int nuj = j >= 0 ? S.nup[j] : 0, ndj = j >= 0 ? S.ndn[j] : 0;
+    const int eui = lane < nui ? S.nb_up[(size_t)i * cap + lane] : 0;
+    const int edi = lane < ndi ? S.nb_dn[(size_t)i * cap + lane] : 0;
+    const int euj = lane < nuj ? S.nb_up[(size_t)j * cap + lane] : 0;
+    const int edj = lane < ndj ? S.nb_dn[(size_t)j * cap + lane] : 0;
+    // ---- work list: candidates of i (tag 0) and j (tag 1), then the down
+    //      set with its op partner (tag 2: i, tag 3: j), in reference order
+    const int xi0 = P.xi.x, xi1 = P.xi.y, xi2 = P.xi.z;
+    const int xj0 = P.xj.x, xj1 = P.xj.y, xj2 = P.xj.z;
+    int w = 0;
+    if (lane < nui) {
+        L.wm[lane] = eui;
+        L.wx[lane] = 0;
     }
-    nbl = j >= 0 ? 2 : 1;
-    // (b) the down set: dnnab(i), extra(i,:)<i, dnnab(j)\{i}, extra(j,:)<j
-    const int cap = S.cap;
-    int nD = 0;
-    const int ndi = S.ndn[i];
-    for (int base = 0; base < ndi; base += kW) {
-        const int p = base + lane;
-        if (p < ndi && p < L.maxd) {
-            L.dl_l[p] = S.nb_dn[(size_t)i * cap + p] & 0xFFFFFF;
-            L.dl_tag[p] = 0;
-        }
-    }
-    nD = ndi;
-    {
-        const int4 e = S.extra[i];
-        const int ev[3] = {e.x, e.y, e.z};
-        for (int k = 0; k < 3; ++k)
-            if (ev[k] >= 0 && ev[k] < i) {
-                if (lane == 0 && nD < L.maxd) {
-                    L.dl_l[nD] = ev[k];
-                    L.dl_tag[nD] = 0;
-                }
-                ++nD;
-            }
-    }
+    w = nui;
+    const int kNoStat = 0xFF << 24;
+    if (xi0 > i) { if (lane == 0) { L.wm[w] = xi0 | kNoStat; L.wx[w] = 0; } ++w; }
+    if (xi1 > i) { if (lane == 0) { L.wm[w] = xi1 | kNoStat; L.wx[w] = 0; } ++w; }
+    if (xi2 > i) { if (lane == 0) { L.wm[w] = xi2 | kNoStat; L.wx[w] = 0; } ++w; }
     if (j >= 0) {
-        const int ndj = S.ndn[j];
-        for (int base = 0; base < ndj; base += kW) {
-            const int p = base + lane;
-            const int l = p < ndj ? (S.nb_dn[(size_t)j * cap + p] & 0xFFFFFF) : -1;
-            const bool keep = p < ndj && l != i;
-            const unsigned long long bal = ballot(keep);
-            const int pre = __popcll(bal & ((1ull << lane) - 1ull));
-            if (keep && nD + pre < L.maxd) {
-                L.dl_l[nD + pre] = l;
-                L.dl_tag[nD + pre] = 1;
-            }
-            nD += __popcll(bal);
+        if (lane < nuj) {
+            L.wm[w + lane] = euj;
+            L.wx[w + lane] = 1;
         }
-        const int4 e = S.extra[j];
-        const int ev[3] = {e.x, e.y, e.z};
-        for (int k = 0; k < 3; ++k)
-            if (ev[k] >= 0 && ev[k] < j) {
-                if (lane == 0 && nD < L.maxd) {
-                    L.dl_l[nD] = ev[k];
-                    L.dl_tag[nD] = 1;
-                }
-                ++nD;
-            }
+        w += nuj;
+        if (xj0 > j) { if (lane == 0) { L.wm[w] = xj0 | kNoStat; L.wx[w] = 1; } ++w; }
+        if (xj1 > j) { if (lane == 0) { L.wm[w] = xj1 | kNoStat; L.wx[w] = 1; } ++w; }
+        if (xj2 > j) { if (lane == 0) { L.wm[w] = xj2 | kNoStat; L.wx[w] = 1; } ++w; }
     }
-    if (nD > L.maxd) {
-        status = kErrListCapacity;
-        nD = L.maxd;
+    const int ncand = w;
+    if (lane < ndi) {
+        L.wm[w + lane] = edi & 0xFFFFFF;
+        L.wx[w + lane] = 2;
     }
+    w += ndi;
+    if (xi0 >= 0 && xi0 < i) { if (lane == 0) { L.wm[w] = xi0; L.wx[w] = 2; } ++w; }
+    if (xi1 >= 0 && xi1 < i) { if (lane == 0) { L.wm[w] = xi1; L.wx[w] = 2; } ++w; }
+    if (xi2 >= 0 && xi2 < i) { if (lane == 0) { L.wm[w] = xi2; L.wx[w] = 2; } ++w; }
+    if (j >= 0) {
+        const int l = edj & 0xFFFFFF;
+        const bool keep = lane < ndj && l != i;
+        const unsigned long long bal = __ballot(keep);
+        const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+        if (keep) {
+            L.wm[w + pre] = l;
+            L.wx[w + pre] = 3;
+        }
+        w += __popcll(bal);
+        if (xj0 >= 0 && xj0 < j) { if (lane == 0) { L.wm[w] = xj0; L.wx[w] = 3; } ++w; }
+        if (xj1 >= 0 && xj1 < j) { if (lane == 0) { L.wm[w] = xj1; L.wx[w] = 3; } ++w; }
+        if (xj2 >= 0 && xj2 < j) { if (lane == 0) { L.wm[w] = xj2; L.wx[w] = 3; } ++w; }
+    }
+    const int nD = w - ncand;
+    if (w > L.maxw) status = kErrListCapacity;
     __syncthreads();
-    // (c) per unique l: apply its I-ops then J-ops in reference order
+    // ---- candidate predictions of the full recomputes of i and j
+    double bt0 = 1e300, bt1 = 1e300;
+    int bp0 = INT_MAX, bp1 = INT_MAX, bm0 = -1, bm1 = -1, bty0 = -1, bty1 = -1;
+    for (int base = 0; base < ncand; base += kW) {
+        const int q = base + lane;
+        if (q < ncand && q < L.maxw) {
+            const int ent = L.wm[q];
+            const int owner = L.wx[q];
+            const int m = ent & 0xFFFFFF;
+            int stat = (ent >> 24) & 0xFF;
+            const Rec rx = owner == 0 ? P.ri : P.rj;
+            const int x = owner == 0 ? i : j;
+            const Rec rm = load_rec(S, P, m);
+            const int4 om = S.ovr[m];
+            if (stat == 0xFF) stat = static_t(rm.topo, rx.topo);
+            const int code = dyn_code(om, x, stat);  // ev_code(m, x)
+            double tij = 1000000000.0;
+            int ty = 0;
+            predict(T, code, rx, rm, dyn_bptnr(rx.dyn) == m, owner == 0 ? P.bwi : P.bwj, tf, tij, ty);
+            if (ty == -99) status = kErrBadCode;
+            if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
+            if (owner == 0) {
+                if (tij < bt0) {
+                    bt0 = tij;
+                    bp0 = q;
+                    bm0 = m;
+                    bty0 = ty;
+                }
+            } else if (tij < bt1) {
+                bt1 = tij;
+                bp1 = q;
+                bm1 = m;
+                bty1 = ty;
+            }
+        }
+    }
+    wave_min_pred(bt0, bp0, bm0, bty0);
+    wave_min_pred(bt1, bp1, bm1, bty1);
+    double ti = T0 + tf, tj = T0 + tf;
+    int pi = -1, ci = -1, pj = -1, cj = -1;
+    if (bt0 < T0) {
+        ti = bt0 + tf;
+        pi = bm0;
+        ci = bty0;
+    }
+    if (bt1 < T0) {
+        tj = bt1 + tf;
+        pj = bm1;
+        cj = bty1;
+    }
+    // ---- the down set: per unique bead, its I-ops then its J-ops
+    if (lane == 0) {
+        L.blist[0] = i / kBlk;
+        L.blist[1] = j >= 0 ? j / kBlk : i / kBlk;
+        L.blist[2] = extra_blk >= 0 ? extra_blk : i / kBlk;
+    }
+    int nbl = 3;
     for (int base = 0; base < nD; base += kW) {
         const int q = base + lane;
-        bool first = false;
+        bool first = false, is_i = false;
         int l = -1, cI = 0, cJ = 0;
-        if (q < nD) {
-            l = L.dl_l[q];
+        if (q < nD && ncand + q < L.maxw) {
+            l = L.wm[ncand + q];
             first = true;
             for (int r = 0; r < q; ++r)
-                if (L.dl_l[r] == l) {
+                if (L.wm[ncand + r] == l) {
                     first = false;
                     break;
                 }
             if (first)
-                for (int r = 0; r < nD; ++r)
-                    if (L.dl_l[r] == l) {
-                        if (L.dl_tag[r] == 0) ++cI;
+                for (int r = 0; r < nD && ncand + r < L.maxw; ++r)
+                    if (L.wm[ncand + r] == l) {
+                        if (L.wx[ncand + r] == 2) ++cI;
                         else ++cJ;
                     }
+            is_i = (l == i);
         }
+        Rec rl = P.ri;
+        int4 ol = make_int4(-1, -1, -1, -1), xl = make_int4(-1, -1, -1, -1);
+        double2 bwl = make_double2(0.0, 0.0);
         double st_t = 0.0;
         int st_p = -1, st_c = -1;
-        bool modified = false, pending = false;
         if (first) {
-            st_t = S.tim[l];
-            st_p = S.nptnr[l];
-            st_c = S.coltype[l];
+            rl = load_rec(S, P, l);
+            ol = S.ovr[l];
+            bwl = S.bw[l];
+            xl = S.extra[l];
+            if (is_i) {  // bead i, reached through extra(j,:): its fresh result
+                st_t = ti;
+                st_p = pi;
+                st_c = ci;
+            } else {
+                const Leaf lf = L.leaf[l];
+                st_t = lf.tim;
+                st_p = lf.nptnr;
+                st_c = leaf_ct(lf.meta);
+            }
         }
+        bool modified = false, pending = false;
         int rem = first ? cI + cJ : 0;
         for (int guard = 0; guard < 64 * 9; ++guard) {
             while (rem > 0 && !pending) {
-                const int p = (rem > cJ) ? i : j;
+                const bool opi = rem > cJ;
+                const int p = opi ? i : j;
                 --rem;
                 if (st_p != p)
-                    single_update(S, T, l, p, tf, st_t, st_p, st_c, modified, status);
+                    single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
                 else
                     pending = true;
             }
-            const unsigned long long pm = ballot(pending);
+            const unsigned long long pm = __ballot(pending);
             if (pm == 0) break;
             const int leader = __ffsll((long long)pm) - 1;
-            const int lf = __shfl(l, leader, kW);
-            double ft, dt;
-            int fp, fc, dp, dc;
-            wave_full2(S, T, lf, -1, tf, T0, ft, fp, fc, dt, dp, dc, status);
+            const int lf_l = __shfl(l, leader, kW);
+            const Rec rlf = shfl_rec(rl, leader);
+            const double2 bwlf = make_double2(__shfl(bwl.x, leader, kW), __shfl(bwl.y, leader, kW));
+            const int4 xlf = make_int4(__shfl(xl.x, leader, kW), __shfl(xl.y, leader, kW), __shfl(xl.z, leader, kW),
+                                       __shfl(xl.w, leader, kW));
+            double ft;
+            int fp, fc;
+            full_one(S, T, P, lf_l, rlf, bwlf, xlf, tf, T0, ft, fp, fc, status);
             if (lane == leader) {
                 st_t = ft;
                 st_p = fp;
@@ -698,26 +801,55 @@ __device__ void partial_events(const SysDev& S, const DevTables& T, const Lds& L
                 pending = false;
             }
         }
-        if (modified) {
-            S.tim[l] = st_t;
-            S.nptnr[l] = st_p;
-            S.coltype[l] = st_c;
-            S.cbin[l] = calc_bin(c, st_t, false, status);
+        // bead i's state lives in (ti, pi, ci) and is written with i's leaf
+        const unsigned long long im = __ballot(first && is_i);
+        if (im) {
+            const int src = __ffsll((long long)im) - 1;
+            ti = __shfl(st_t, src, kW);
+            pi = __shfl(st_p, src, kW);
+            ci = __shfl(st_c, src, kW);
         }
-        const unsigned long long mm = ballot(modified);
+        if (is_i) modified = false;
+        if (modified) {
+            Leaf lf;
+            lf.tim = st_t;
+            lf.nptnr = st_p;
+            lf.meta = make_meta(calc_bin(c, st_t, false, status), st_c);
+            L.leaf[l] = lf;
+        }
+        const unsigned long long mm = __ballot(modified);
         const int pre = __popcll(mm & ((1ull << lane) - 1ull));
-        if (modified && nbl + pre < L.maxblist) L.blist[nbl + pre] = l / kBlk;
+        if (modified && nbl + pre < kMaxBlist) L.blist[nbl + pre] = l / kBlk;
         nbl += __popcll(mm);
-        if (nbl > L.maxblist) nbl = L.maxblist;
+        if (nbl > kMaxBlist) nbl = kMaxBlist;
     }
-    // (d) repuls_del_b.f once the rescheduling is done
+    if (lane == 0) {
+        Leaf a;
+        a.tim = ti;
+        a.nptnr = pi;
+        a.meta = make_meta(calc_bin(c, ti, false, status), ci);
+        L.leaf[i] = a;
+        if (j >= 0) {
+            Leaf b;
+            b.tim = tj;
+            b.nptnr = pj;
+            b.meta = make_meta(calc_bin(c, tj, false, status), cj);
+            L.leaf[j] = b;
+        }
+    }
+    // repuls_del_b.f once the rescheduling is done
     if (xpulse_del && lane == 0) {
-        const int a = S.ident[i] < S.ident[j] ? i : j, b = a == i ? j : i;
-        const int ncim1 = a + S.clen[a] - 1, ncai = a - S.clen[a];
-        const int ncaj = b - 2 * S.clen[b], nnjp1 = b - S.clen[b] + 1;
+        const bool iN = dyn_ident(P.ri.dyn) < dyn_ident(P.rj.dyn);
+        const int a = iN ? i : j, b = iN ? j : i;
+        const int La = topo_len(iN ? P.ri.topo : P.rj.topo), Lb = topo_len(iN ? P.rj.topo : P.ri.topo);
+        const int ncim1 = a + La - 1, ncai = a - La, ncaj = b - 2 * Lb, nnjp1 = b - Lb + 1;
         int4 ea = S.extra[a], eb = S.extra[b];
-        ea.x = -1; ea.y = -1; ea.w = -1;
-        eb.x = -1; eb.y = -1; eb.w = -1;
+        ea.x = -1;
+        ea.y = -1;
+        ea.w = -1;
+        eb.x = -1;
+        eb.y = -1;
+        eb.w = -1;
         S.extra[a] = ea;
         S.extra[b] = eb;
         S.extra[ncaj].z = -1;
@@ -727,25 +859,35 @@ __device__ void partial_events(const SysDev& S, const DevTables& T, const Lds& L
     }
     status = wave_or(status);
     if (status && !c.status) c.status = status;
+    if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
-    rescan_blocks(S, L, N3, nbl);
+    rescan_block_list(L, N3, nbl);
 }
 
 // ------------------------------------------------------------------------
-// event dynamics: eventdyn.f, bumped.f, repuls_*.f (single lane)
+// event dynamics: eventdyn.f, bumped.f, repuls_*.f (uniform over lanes)
 // ------------------------------------------------------------------------
-__device__ inline void nudge(Bead& bi, Bead& bj, double bd, const Geo& g, int sign) {
+__device__ inline void nudge(Rec& bi, Rec& bj, double bd, const Geo& g, int sign) {
     if (sign > 0) {
-        bi.x = bi.x + bd * g.rx; bi.y = bi.y + bd * g.ry; bi.z = bi.z + bd * g.rz;
-        bj.x = bj.x - bd * g.rx; bj.y = bj.y - bd * g.ry; bj.z = bj.z - bd * g.rz;
+        bi.x = bi.x + bd * g.rx;
+        bi.y = bi.y + bd * g.ry;
+        bi.z = bi.z + bd * g.rz;
+        bj.x = bj.x - bd * g.rx;
+        bj.y = bj.y - bd * g.ry;
+        bj.z = bj.z - bd * g.rz;
     } else {
-        bi.x = bi.x - bd * g.rx; bi.y = bi.y - bd * g.ry; bi.z = bi.z - bd * g.rz;
-        bj.x = bj.x + bd * g.rx; bj.y = bj.y + bd * g.ry; bj.z = bj.z + bd * g.rz;
+        bi.x = bi.x - bd * g.rx;
+        bi.y = bi.y - bd * g.ry;
+        bi.z = bi.z - bd * g.rz;
+        bj.x = bj.x + bd * g.rx;
+        bj.y = bj.y + bd * g.ry;
+        bj.z = bj.z + bd * g.rz;
     }
 }
 
-__device__ void eventdyn(const SysDev& S, const DevTables& T, Bead& bi, Bead& bj, int i, int j, int a, int c,
-                         int evcode, double tf, int& ct, int& status) {
+__device__ void eventdyn(const DevTables& T, Rec& bi, Rec& bj, double2 bwi, bool bonded_ij, int evcode, double tf,
+                         int& ct, int& status) {
+    const int a = dyn_ident(bi.dyn), c = dyn_ident(bj.dyn);
     Geo g;
     geo(bi, bj, tf, g);
     const double bmi = bi.bm, bmj = bj.bm;
@@ -755,14 +897,14 @@ __device__ void eventdyn(const SysDev& S, const DevTables& T, Bead& bi, Bead& bj
         case 2:
         case 3: {
             double blmin, blmax;
-            bond_window(S, T, evcode, S.res[i], blmin, blmax);
+            bond_window(T, evcode, bwi, blmin, blmax);
             ratio = ct == 2 ? rmass * g.b / (blmin * blmin) : rmass * g.b / (blmax * blmax);
             break;
         }
         case 1: {
             double sigsq;
             if (evcode == 15) {
-                if (S.bptnr[i] == j) {
+                if (bonded_ij) {
                     const double pe = T.ev_param[1][evcode];
                     sigsq = T.sigma_sq[a][c] * (pe * pe);
                 } else
@@ -865,90 +1007,107 @@ __device__ void eventdyn(const SysDev& S, const DevTables& T, Bead& bi, Bead& bj
     bj.z = bj.z - dvz * tf / bmj;
 }
 
-__device__ inline double dist_tf(const SysDev& S, int a, int b, double tf) {
+__device__ inline double dist_rec(const Rec& a, const Rec& b, double tf) {
     Geo g;
-    const Bead ba = S.bead[a], bb = S.bead[b];
-    geo(ba, bb, tf, g);
+    geo(a, b, tf, g);
     return sqrt(g.rx * g.rx + g.ry * g.ry + g.rz * g.rz);
 }
-__device__ inline void shoulder_idx(const SysDev& S, int i, int j, int& ncim1, int& ncai, int& ncaj, int& nnjp1) {
-    ncim1 = i + S.clen[i] - 1;
-    ncai = i - S.clen[i];
-    ncaj = j - 2 * S.clen[j];
-    nnjp1 = j - S.clen[j] + 1;
+__device__ inline void shoulder_idx(int i, int Li, int j, int Lj, int& ncim1, int& ncai, int& ncaj, int& nnjp1) {
+    ncim1 = i + Li - 1;
+    ncai = i - Li;
+    ncaj = j - 2 * Lj;
+    nnjp1 = j - Lj + 1;
 }
-__device__ double repuls_check(const SysDev& S, const DevTables& T, int i, int j, double tf) {
+// repuls_check.f:16-81 (n = the N bead, cb = the C bead)
+__device__ double repuls_check(const SysDev& S, const DevTables& T, const Pair& P, int n, int cb, double tf) {
+    const Rec rn = load_rec(S, P, n), rc = load_rec(S, P, cb);
     int ncim1, ncai, ncaj, nnjp1;
-    shoulder_idx(S, i, j, ncim1, ncai, ncaj, nnjp1);
-    const double d1 = dist_tf(S, i, ncaj, tf), d2 = dist_tf(S, i, nnjp1, tf);
-    const double d3 = dist_tf(S, j, ncai, tf), d4 = dist_tf(S, j, ncim1, tf);
+    shoulder_idx(n, topo_len(rn.topo), cb, topo_len(rc.topo), ncim1, ncai, ncaj, nnjp1);
+    const double d1 = dist_rec(rn, load_rec(S, P, ncaj), tf), d2 = dist_rec(rn, load_rec(S, P, nnjp1), tf);
+    const double d3 = dist_rec(rc, load_rec(S, P, ncai), tf), d4 = dist_rec(rc, load_rec(S, P, ncim1), tf);
     if (d1 > T.shder[1] && d2 > T.shder[2] && d3 > T.shder[3] && d4 > T.shder[4]) return 10.0;
     return 11.0;
 }
-__device__ double repuls_check_3(const SysDev& S, const DevTables& T, int i, int j, int k, double tf) {
+// repuls_check_3.f:16-102
+__device__ double repuls_check_3(const SysDev& S, const DevTables& T, const Pair& P, int n, int cb, int k,
+                                 double tf) {
+    const Rec rn = load_rec(S, P, n), rc = load_rec(S, P, cb);
     int ncim1, ncai, ncaj, nnjp1;
-    shoulder_idx(S, i, j, ncim1, ncai, ncaj, nnjp1);
+    shoulder_idx(n, topo_len(rn.topo), cb, topo_len(rc.topo), ncim1, ncai, ncaj, nnjp1);
     int m = 0;
-    if (ncaj != k && dist_tf(S, i, ncaj, tf) > T.shder[1]) ++m;
-    if (nnjp1 != k && dist_tf(S, i, nnjp1, tf) > T.shder[2]) ++m;
-    if (ncai != k && dist_tf(S, j, ncai, tf) > T.shder[3]) ++m;
-    if (ncim1 != k && dist_tf(S, j, ncim1, tf) > T.shder[4]) ++m;
+    if (ncaj != k && dist_rec(rn, load_rec(S, P, ncaj), tf) > T.shder[1]) ++m;
+    if (nnjp1 != k && dist_rec(rn, load_rec(S, P, nnjp1), tf) > T.shder[2]) ++m;
+    if (ncai != k && dist_rec(rc, load_rec(S, P, ncai), tf) > T.shder[3]) ++m;
+    if (ncim1 != k && dist_rec(rc, load_rec(S, P, ncim1), tf) > T.shder[4]) ++m;
     return m == 3 ? 10.0 : 11.0;
 }
-__device__ void repuls_add(const SysDev& S, int i, int j, int& status) {
+// repuls_add.f:14-47 / repuls_del_a.f:14-37 (single lane; n = N bead, cb = C bead)
+__device__ void repuls_add(const SysDev& S, int n, int Ln, int cb, int Lc, int& status) {
     int ncim1, ncai, ncaj, nnjp1;
-    shoulder_idx(S, i, j, ncim1, ncai, ncaj, nnjp1);
-    set_code(S, ncaj, i, 50, status);
-    set_code(S, i, ncaj, 40, status);
-    set_code(S, i, nnjp1, 40, status);
-    set_code(S, nnjp1, i, 50, status);
-    set_code(S, ncai, j, 50, status);
-    set_code(S, j, ncai, 40, status);
-    set_code(S, j, ncim1, 40, status);
-    set_code(S, ncim1, j, 50, status);
-    int4 ei = S.extra[i], ej = S.extra[j];
-    ei.x = ncaj; ei.y = nnjp1; ei.w = j;
-    ej.x = ncai; ej.y = ncim1; ej.w = i;
-    S.extra[i] = ei;
-    S.extra[j] = ej;
-    S.extra[ncaj].z = i;
-    S.extra[nnjp1].z = i;
-    S.extra[ncai].z = j;
-    S.extra[ncim1].z = j;
+    shoulder_idx(n, Ln, cb, Lc, ncim1, ncai, ncaj, nnjp1);
+    set_code(S, ncaj, n, 50, status);
+    set_code(S, n, ncaj, 40, status);
+    set_code(S, n, nnjp1, 40, status);
+    set_code(S, nnjp1, n, 50, status);
+    set_code(S, ncai, cb, 50, status);
+    set_code(S, cb, ncai, 40, status);
+    set_code(S, cb, ncim1, 40, status);
+    set_code(S, ncim1, cb, 50, status);
+    int4 en = S.extra[n], ec = S.extra[cb];
+    en.x = ncaj;
+    en.y = nnjp1;
+    en.w = cb;
+    ec.x = ncai;
+    ec.y = ncim1;
+    ec.w = n;
+    S.extra[n] = en;
+    S.extra[cb] = ec;
+    S.extra[ncaj].z = n;
+    S.extra[nnjp1].z = n;
+    S.extra[ncai].z = cb;
+    S.extra[ncim1].z = cb;
 }
-__device__ void repuls_del_a(const SysDev& S, int i, int j, int& status) {
+__device__ void repuls_del_a(const SysDev& S, int n, int Ln, int cb, int Lc, int& status) {
     int ncim1, ncai, ncaj, nnjp1;
-    shoulder_idx(S, i, j, ncim1, ncai, ncaj, nnjp1);
-    set_code(S, ncaj, i, 1, status);
-    set_code(S, i, ncaj, 1, status);
-    set_code(S, i, nnjp1, 1, status);
-    set_code(S, nnjp1, i, 1, status);
-    set_code(S, ncai, j, 1, status);
-    set_code(S, j, ncai, 1, status);
-    set_code(S, ncim1, j, 1, status);
-    set_code(S, j, ncim1, 1, status);
+    shoulder_idx(n, Ln, cb, Lc, ncim1, ncai, ncaj, nnjp1);
+    set_code(S, ncaj, n, 1, status);
+    set_code(S, n, ncaj, 1, status);
+    set_code(S, n, nnjp1, 1, status);
+    set_code(S, nnjp1, n, 1, status);
+    set_code(S, ncai, cb, 1, status);
+    set_code(S, cb, ncai, 1, status);
+    set_code(S, ncim1, cb, 1, status);
+    set_code(S, cb, ncim1, 1, status);
 }
-__device__ inline bool terminal(const SysDev& S, int k) {
-    return S.loc[k] == S.clen[k] + 1 || S.loc[k] == 3 * S.clen[k];
+__device__ inline bool terminal_t(int t) {
+    const int k = topo_kind(t), r = topo_res(t), L = topo_len(t);
+    return (k == 1 && r == 1) || (k == 2 && r == L);
 }
 
 // One pair event: outcome resolution (main.F90:1487-1634), dynamics, H-bond
 // and shoulder bookkeeping (main.F90:641-921), then partial_events.
-__device__ int pair_event(const SysDev& S, const DevTables& T, const Lds& L, Sc& c, int i) {
+__device__ int pair_event(const SysDev& S, const DevTables& T, const Lds& L, Sc& c, int i, int j, int ct) {
     const int lane = threadIdx.x;
-    const int j = S.nptnr[i];
-    int ct = S.coltype[i];
     const double tf = c.tfalse;
-    const int evcode = dyn_code(S, i, j, static_code(S, i, j));
-    const bool ok = !terminal(S, i) && !terminal(S, j);
+    Pair P;
+    P.i = i;
+    P.j = j;
+    P.ri = S.rec[i];
+    P.rj = S.rec[j];
+    P.bwi = S.bw[i];
+    P.bwj = S.bw[j];
+    P.xi = S.extra[i];
+    P.xj = S.extra[j];
+    const int4 oi = S.ovr[i];
+    const int evcode = dyn_code(oi, j, static_t(P.ri.topo, P.rj.topo));  // ev_code(i, j)
+    const bool ok = !terminal_t(P.ri.topo) && !terminal_t(P.rj.topo);
+    const int idi = dyn_ident(P.ri.dyn), idj = dyn_ident(P.rj.dyn);
     int status = 0;
-    bool xpulse = false;
-    const int idi = S.ident[i], idj = S.ident[j];
+    bool xpulse = false, fence = false;
     if (ct == 7) {
-        const int4 ei = S.extra[i], ej = S.extra[j];
-        if (ei.w < 0 && ej.w < 0) {
+        if (P.xi.w < 0 && P.xj.w < 0) {
             if (ok) {
-                const double rating = idi < idj ? repuls_check(S, T, i, j, tf) : repuls_check(S, T, j, i, tf);
+                const double rating = idi < idj ? repuls_check(S, T, P, i, j, tf) : repuls_check(S, T, P, j, i, tf);
                 ct = rating <= 10.0 ? 4 : 14;
             } else {
                 const double ran_non = rng_u01(S.seed, 1, c.rc1++);
@@ -959,98 +1118,122 @@ __device__ int pair_event(const SysDev& S, const DevTables& T, const Lds& L, Sc&
     } else if (ct == 10) {
         double rating;
         if (evcode < 45) {
-            const int hbp = S.extra[i].w;
-            rating = idi < S.ident[hbp] ? repuls_check_3(S, T, i, hbp, j, tf) : repuls_check_3(S, T, hbp, i, j, tf);
+            const int hbp = P.xi.w;
+            const int idh = dyn_ident(load_rec(S, P, hbp).dyn);
+            rating = idi < idh ? repuls_check_3(S, T, P, i, hbp, j, tf) : repuls_check_3(S, T, P, hbp, i, j, tf);
         } else {
-            const int hbp = S.extra[j].w;
-            rating = idj < S.ident[hbp] ? repuls_check_3(S, T, j, hbp, i, tf) : repuls_check_3(S, T, hbp, j, i, tf);
+            const int hbp = P.xj.w;
+            const int idh = dyn_ident(load_rec(S, P, hbp).dyn);
+            rating = idj < idh ? repuls_check_3(S, T, P, j, hbp, i, tf) : repuls_check_3(S, T, P, hbp, j, i, tf);
         }
         ct = rating <= 10.0 ? 5 : 15;
     } else if (ct == 12) {
-        const int x = evcode < 45 ? i : j;
-        const int hbp = S.extra[x].w;
-        if (S.bptnr[x] == hbp) {
-            const double d2 = [&] {
-                Geo g;
-                const Bead bx = S.bead[x], bh = S.bead[hbp];
-                geo(bx, bh, tf, g);
-                return g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
-            }();
-            const double rating = (d2 - T.sigma_sq[S.ident[x]][S.ident[hbp]]) < 0.0 ? 10.0 : 11.0;
+        const bool xi_ = evcode < 45;
+        const Rec rx = xi_ ? P.ri : P.rj;
+        const int hbp = xi_ ? P.xi.w : P.xj.w;
+        if (dyn_bptnr(rx.dyn) == hbp) {
+            Geo g;
+            const Rec rh = load_rec(S, P, hbp);
+            geo(rx, rh, tf, g);
+            const double d2 = g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
+            const double rating = (d2 - T.sigma_sq[dyn_ident(rx.dyn)][dyn_ident(rh.dyn)]) < 0.0 ? 10.0 : 11.0;
             ct = rating > 10.0 ? 6 : 13;
         } else
             ct = 15;
     }
-    Bead bi = S.bead[i], bj = S.bead[j];
-    if (ct < 14) eventdyn(S, T, bi, bj, i, j, idi, idj, evcode, tf, ct, status);
-    __syncthreads();
-    if (lane == 0) {
-        if (ct == 20) {
-            if (idi + idj == 5) {
-                S.bptnr[i] = j;
-                S.bptnr[j] = i;
-                S.ident[i] = idi + 4;
-                S.ident[j] = idj + 4;
-                if (ok) {
-                    if (idi + 4 < idj + 4) repuls_add(S, i, j, status);
-                    else repuls_add(S, j, i, status);
+    if (ct < 14) eventdyn(T, P.ri, P.rj, P.bwi, dyn_bptnr(P.ri.dyn) == j, evcode, tf, ct, status);
+    const int Li = topo_len(P.ri.topo), Lj = topo_len(P.rj.topo);
+    if (ct == 20) {
+        if (idi + idj == 5) {
+            P.ri.dyn = make_dyn(idi + 4, j);
+            P.rj.dyn = make_dyn(idj + 4, i);
+            if (ok) {
+                if (lane == 0) {
+                    if (idi < idj) repuls_add(S, i, Li, j, Lj, status);
+                    else repuls_add(S, j, Lj, i, Li, status);
                 }
-            }
-        } else if (ct == 21) {
-            if (idi <= 8) {
-                if (ok) {
-                    if (idi < idj) repuls_del_a(S, i, j, status);
-                    else repuls_del_a(S, j, i, status);
-                    xpulse = true;
-                }
-                if (S.bptnr[i] == j) {
-                    S.bptnr[i] = -1;
-                    S.bptnr[j] = -1;
-                    S.ident[i] = idi - 4;
-                    S.ident[j] = idj - 4;
-                }
-            }
-        } else if (ct == 24) {
-            const int x = evcode < 45 ? i : j;
-            const int hbp = S.extra[x].w;
-            if (S.ident[x] + S.ident[hbp] == 5) {
-                S.bptnr[x] = hbp;
-                S.bptnr[hbp] = x;
-                S.ident[x] += 4;
-                S.ident[hbp] += 4;
-            }
-        } else if (ct == 25) {
-            const int x = evcode < 45 ? i : j;
-            const int hbp = S.extra[x].w;
-            if (S.ident[x] >= 5) {
-                S.bptnr[x] = -1;
-                S.bptnr[hbp] = -1;
-                S.ident[x] -= 4;
-                S.ident[hbp] -= 4;
-            }
-        } else if (ct == 14 || ct == 15 || ct == 16) {
-            // bumped.f: push the pair off the discontinuity
-            Geo g;
-            geo(bi, bj, tf, g);
-            const double bd = evcode >= 40 ? kSm * sqrt(T.shlddia_sq[idi][idj]) : kSm * sqrt(T.welldia_sq[idi][idj]);
-            nudge(bi, bj, bd, g, g.b < 0.0 ? -1 : +1);
-            if (ct == 14 && ok) {
-                if (idi < idj) repuls_add(S, i, j, status);
-                else repuls_add(S, j, i, status);
-            } else if (ct == 16 && ok) {
-                if (idi < idj) repuls_del_a(S, i, j, status);
-                else repuls_del_a(S, j, i, status);
-                xpulse = true;
+                fence = true;
             }
         }
-        S.bead[i] = bi;
-        S.bead[j] = bj;
+    } else if (ct == 21) {
+        if (idi <= 8) {
+            if (ok) {
+                if (lane == 0) {
+                    if (idi < idj) repuls_del_a(S, i, Li, j, Lj, status);
+                    else repuls_del_a(S, j, Lj, i, Li, status);
+                }
+                xpulse = true;
+                fence = true;
+            }
+            if (dyn_bptnr(P.ri.dyn) == j) {
+                P.ri.dyn = make_dyn(idi - 4, -1);
+                P.rj.dyn = make_dyn(idj - 4, -1);
+            }
+        }
+    } else if (ct == 24 || ct == 25) {
+        const bool xi_ = evcode < 45;
+        const int hbp = xi_ ? P.xi.w : P.xj.w;
+        const int x = xi_ ? i : j;
+        int xdyn = xi_ ? P.ri.dyn : P.rj.dyn;
+        int hdyn = load_rec(S, P, hbp).dyn;
+        const int idx = dyn_ident(xdyn), idh = dyn_ident(hdyn);
+        bool changed = false;
+        if (ct == 24 && idx + idh == 5) {
+            xdyn = make_dyn(idx + 4, hbp);
+            hdyn = make_dyn(idh + 4, x);
+            changed = true;
+        } else if (ct == 25 && idx >= 5) {
+            xdyn = make_dyn(idx - 4, -1);
+            hdyn = make_dyn(idh - 4, -1);
+            changed = true;
+        }
+        if (changed) {
+            if (xi_) P.ri.dyn = xdyn;
+            else P.rj.dyn = xdyn;
+            if (hbp == i) P.ri.dyn = hdyn;
+            else if (hbp == j) P.rj.dyn = hdyn;
+            else {
+                if (lane == 0) S.rec[hbp].dyn = hdyn;
+                fence = true;
+            }
+        }
+    } else if (ct == 14 || ct == 15 || ct == 16) {
+        // bumped.f: push the pair off the discontinuity
+        Geo g;
+        geo(P.ri, P.rj, tf, g);
+        const double bd = evcode >= 40 ? kSm * sqrt(T.shlddia_sq[idi][idj]) : kSm * sqrt(T.welldia_sq[idi][idj]);
+        nudge(P.ri, P.rj, bd, g, g.b < 0.0 ? -1 : +1);
+        if (ct == 14 && ok) {
+            if (lane == 0) {
+                if (idi < idj) repuls_add(S, i, Li, j, Lj, status);
+                else repuls_add(S, j, Lj, i, Li, status);
+            }
+            fence = true;
+        } else if (ct == 16 && ok) {
+            if (lane == 0) {
+                if (idi < idj) repuls_del_a(S, i, Li, j, Lj, status);
+                else repuls_del_a(S, j, Lj, i, Li, status);
+            }
+            xpulse = true;
+            fence = true;
+        }
     }
-    xpulse = __shfl(xpulse ? 1 : 0, 0, kW) != 0;
+    if (lane == 0) {
+        S.rec[i] = P.ri;
+        S.rec[j] = P.rj;
+    }
+    if (fence) {
+        // shoulder tables changed: make them visible before the predictions
+        __threadfence_block();
+        __syncthreads();
+        P.xi = S.extra[i];
+        P.xj = S.extra[j];
+    }
     status = wave_or(status);
     if (status && !c.status) c.status = status;
+    if (lane == 0) *L.pair = P;
     __syncthreads();
-    partial_events(S, T, L, c, i, j, xpulse);
+    reschedule(S, T, L, c, *L.pair, -1, xpulse);
     return ct;
 }
 
@@ -1063,13 +1246,21 @@ __device__ int ghost_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
         i = (int)(rng_u01(S.seed, 0, c.rc0++) * N);
     } while (i == N && ++guard < 1000);
     c.numghosts++;
-    Bead b = S.bead[i];
+    Pair P;
+    P.i = i;
+    P.j = -1;
+    P.ri = S.rec[i];
+    P.bwi = S.bw[i];
+    P.xi = S.extra[i];
+    P.rj = P.ri;
+    P.bwj = P.bwi;
+    P.xj = make_int4(-1, -1, -1, -1);
+    Rec b = P.ri;
     const double tf = c.tfalse, bm = b.bm;
     b.x = b.x + b.vx * tf;
     b.y = b.y + b.vy * tf;
     b.z = b.z + b.vz * tf;
     double v1, v2, r, fact;
-    guard = 0;
     do {
         v1 = 2.0 * rng_u01(S.seed, 0, c.rc0++) - 1.0;
         v2 = 2.0 * rng_u01(S.seed, 0, c.rc0++) - 1.0;
@@ -1095,54 +1286,54 @@ __device__ int ghost_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
     if (guard >= 1000) c.status = kErrRng;
     const double tg = -1.0 * dlog(tgho) * S.avegtime + tf;
     int status = 0;
+    P.ri = b;
+    P.rj = b;
     if (lane == 0) {
-        S.bead[i] = b;
-        S.tim[N] = tg;
-        S.cbin[N] = calc_bin(c, tg, false, status);
+        S.rec[i] = b;
+        Leaf g;
+        g.tim = tg;
+        g.nptnr = -2;
+        g.meta = make_meta(calc_bin(c, tg, false, status), -2);
+        L.leaf[N] = g;
     }
     if (status && !c.status) c.status = status;
     if (c.tfalse < c.old_tfalse) c.tfalse = c.old_tfalse;
+    if (lane == 0) *L.pair = P;
     __syncthreads();
-    if (lane == 0) L.blist[0] = N / kBlk;
-    __syncthreads();
-    rescan_blocks(S, L, S.N + 3, 1);
-    partial_events(S, T, L, c, i, -1, false);
+    reschedule(S, T, L, c, *L.pair, N / kBlk, false);
     return i;
 }
 
 // ------------------------------------------------------------------------
 // neighbour lists: nbor.f membership restated for a coarse-cell search
 // ------------------------------------------------------------------------
-// Pair (i, j) is listed when its fine cells (the reference's cell_add.f
-// binning) are within the n_wrap=2 stencil (|d| <= 2 per axis, periodic),
-// and either its code is a bond/local one (4-12, 17-26) or r^2 <= rlsq(code).
-__device__ void rebuild_lists(const SysDev& S, const DevTables& T, const Lds& L, Sc& c) {
+// Pair (i, j) is listed when its fine cells (cell_add.f binning) are within
+// the n_wrap=2 stencil (|d| <= 2 per axis, periodic) and either its code is
+// a bond/local one (4-12, 17-26) or r^2 <= rlsq(code) (nbor.f:63-128).
+__device__ void rebuild_lists(const SysDev& S, const DevTables& T, Sc& c) {
     const int lane = threadIdx.x;
     const int N = S.N, M = S.M, Mq = S.Mq, NC = S.ncoarse, cap = S.cap;
-    // fine and coarse cells
     for (int k = lane; k < N; k += kW) {
-        const Bead b = S.bead[k];
-        int cx = (int)((b.x + S.half) / S.width) + 2 - 2;
-        int cy = (int)((b.y + S.half) / S.width) + 2 - 2;
-        int cz = (int)((b.z + S.half) / S.width) + 2 - 2;
+        const Rec b = S.rec[k];
+        int cx = (int)((b.x + S.half) / S.width);
+        int cy = (int)((b.y + S.half) / S.width);
+        int cz = (int)((b.z + S.half) / S.width);
         cx = ((cx % M) + M) % M;
         cy = ((cy % M) + M) % M;
         cz = ((cz % M) + M) % M;
         S.cfine[k] = cx | (cy << 10) | (cz << 20);
-        int qx = cx >> 1, qy = cy >> 1, qz = cz >> 1;
-        if (qx >= Mq) qx = Mq - 1;
-        if (qy >= Mq) qy = Mq - 1;
-        if (qz >= Mq) qz = Mq - 1;
+        const int qx = min(cx >> 1, Mq - 1), qy = min(cy >> 1, Mq - 1), qz = min(cz >> 1, Mq - 1);
         S.ckey[k] = qx + Mq * (qy + Mq * qz);
     }
     for (int q = lane; q < NC; q += kW) {
         S.cstart[q] = 0;
         S.crun[q] = 0;
     }
+    __threadfence_block();
     __syncthreads();
     for (int k = lane; k < N; k += kW) atomicAdd(&S.cstart[S.ckey[k]], 1);
+    __threadfence_block();
     __syncthreads();
-    // exclusive scan of cstart (wave-wide, chunked)
     int carry = 0;
     for (int base = 0; base < NC; base += kW) {
         const int q = base + lane;
@@ -1156,6 +1347,7 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, const Lds& L,
         carry += __shfl(incl, kW - 1, kW);
     }
     if (lane == 0) S.cstart[NC] = carry;
+    __threadfence_block();
     __syncthreads();
     // stable scatter in bead order
     for (int base = 0; base < N; base += kW) {
@@ -1169,35 +1361,30 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, const Lds& L,
                 ++cnt;
             }
         }
-        int basepos = 0;
-        if (k < N) basepos = S.cstart[key] + S.crun[key];
+        const int basepos = k < N ? S.cstart[key] + S.crun[key] : 0;
+        __threadfence_block();
         __syncthreads();
         if (k < N) {
             S.csorted[basepos + rank] = k;
             if (rank == cnt - 1) S.crun[key] += cnt;
         }
+        __threadfence_block();
         __syncthreads();
     }
-    // per-bead scan of the coarse neighbourhood
-    int mx = 0;
-    int status = 0;
+    int mx = 0, status = 0;
+    const int noff = Mq > 2 ? 3 : Mq;  // distinct coarse offsets {0, +1, -1}
     for (int i = lane; i < N; i += kW) {
         const int fi = S.cfine[i];
         const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
         const int k = S.ckey[i];
         const int qx = k % Mq, qy = (k / Mq) % Mq, qz = k / (Mq * Mq);
-        const Bead bi = S.bead[i];
+        const Rec bi = S.rec[i];
         int nu = 0, nd = 0;
-        int offs[3];
-        int noff = 0;
-        offs[noff++] = 0;
-        if (Mq > 1) offs[noff++] = 1;
-        if (Mq > 2) offs[noff++] = -1;
         for (int a = 0; a < noff; ++a)
             for (int bb = 0; bb < noff; ++bb)
                 for (int cc = 0; cc < noff; ++cc) {
-                    const int nx = (qx + offs[a] + Mq) % Mq, ny = (qy + offs[bb] + Mq) % Mq,
-                              nz = (qz + offs[cc] + Mq) % Mq;
+                    const int oa = a == 2 ? -1 : a, ob = bb == 2 ? -1 : bb, oc = cc == 2 ? -1 : cc;
+                    const int nx = (qx + oa + Mq) % Mq, ny = (qy + ob + Mq) % Mq, nz = (qz + oc + Mq) % Mq;
                     const int cell = nx + Mq * (ny + Mq * nz);
                     const int s0 = S.cstart[cell], s1 = S.cstart[cell + 1];
                     for (int p = s0; p < s1; ++p) {
@@ -1208,16 +1395,14 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, const Lds& L,
                         dx = ((dx % M) + M) % M;
                         dy = ((dy % M) + M) % M;
                         dz = ((dz % M) + M) % M;
-                        if (dx > 2 && dx < M - 2) continue;
-                        if (dy > 2 && dy < M - 2) continue;
-                        if (dz > 2 && dz < M - 2) continue;
-                        const int stat = static_code(S, jb, i);
-                        const int code = dyn_code(S, jb, i, stat);  // ev_code(j,i), nbor.f:63
+                        if ((dx > 2 && dx < M - 2) || (dy > 2 && dy < M - 2) || (dz > 2 && dz < M - 2)) continue;
+                        const Rec bj = S.rec[jb];
+                        const int stat = static_t(bj.topo, bi.topo);
+                        const int code = dyn_code(S.ovr[jb], i, stat);  // ev_code(j,i), nbor.f:63
                         bool take;
                         if ((code >= 4 && code <= 12) || (code >= 17 && code < 27)) {
                             take = true;
                         } else {
-                            const Bead bj = S.bead[jb];
                             double rx = bi.x - bj.x, ry = bi.y - bj.y, rz = bi.z - bj.z;
                             rx = rx - round(rx);
                             ry = ry - round(ry);
@@ -1242,8 +1427,9 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, const Lds& L,
     }
     status = wave_or(status);
     if (status && !c.status) c.status = status;
-    for (int s = 1; s < kW; s <<= 1) mx = max(mx, __shfl_xor(mx, s, kW));
+    mx = wave_imax(mx);
     if (lane == 0 && mx > S.st->max_nb_seen) S.st->max_nb_seen = mx;
+    __threadfence_block();
     __syncthreads();
 }
 
@@ -1252,17 +1438,21 @@ __device__ void events_all(const SysDev& S, const DevTables& T, const Lds& L, Sc
     const int lane = threadIdx.x;
     const int N = S.N, cap = S.cap;
     int status = 0;
+    c.nbin = 1;
     for (int i = lane; i < N; i += kW) {
+        const Rec bi = S.rec[i];
+        const double2 bwi = S.bw[i];
         double ti = c.interval_max + kLt;
         int pi = -1, ci = -1;
         const int n = S.nup[i];
         for (int k = 0; k < n; ++k) {
             const int ent = S.nb_up[(size_t)i * cap + k];
             const int m = ent & 0xFFFFFF;
-            const int code = dyn_code(S, m, i, ent >> 24);
-            double tij;
-            int ty;
-            predict_pair(S, T, i, m, code, 0.0, tij, ty);
+            const Rec bm = S.rec[m];
+            const int code = dyn_code(S.ovr[m], i, ent >> 24);
+            double tij = 1000000000.0;
+            int ty = 0;
+            predict(T, code, bi, bm, dyn_bptnr(bi.dyn) == m, bwi, 0.0, tij, ty);
             if (ty == -99) status = kErrBadCode;
             if (tij < ti) {
                 ti = tij;
@@ -1272,17 +1462,15 @@ __device__ void events_all(const SysDev& S, const DevTables& T, const Lds& L, Sc
             if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
         }
         const int4 e = S.extra[i];
-        const int ev[3] = {e.x, e.y, e.z};
         for (int k = 0; k < 3; ++k) {
-            const int m = ev[k];
+            const int m = k == 0 ? e.x : (k == 1 ? e.y : e.z);
             if (m > i) {
-                const int code = dyn_code(S, m, i, static_code(S, m, i));
+                const Rec bm = S.rec[m];
+                const int code = dyn_code(S.ovr[m], i, static_t(bm.topo, bi.topo));
                 double tij = 1000000000.0;
                 int ty = 0;
-                const Bead bi = S.bead[i], bm = S.bead[m];
                 // events.f:79-83: code 1 -> core, anything else -> shoulder
-                predict(S, T, code == 1 ? 1 : 40, bi, bm, S.ident[i], S.ident[m], S.bptnr[i] == m, S.res[i], 0.0,
-                        tij, ty);
+                predict(T, code == 1 ? 1 : 40, bi, bm, dyn_bptnr(bi.dyn) == m, bwi, 0.0, tij, ty);
                 if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
                 if (tij < ti) {
                     pi = m;
@@ -1291,31 +1479,37 @@ __device__ void events_all(const SysDev& S, const DevTables& T, const Lds& L, Sc
                 }
             }
         }
-        S.tim[i] = ti;
-        S.nptnr[i] = pi;
-        S.coltype[i] = ci;
+        Leaf lf;
+        lf.tim = ti;
+        lf.nptnr = pi;
+        lf.meta = make_meta(calc_bin(c, ti, false, status), ci);
+        L.leaf[i] = lf;
     }
-    __syncthreads();
-    c.nbin = 1;
-    for (int e = lane; e < N + 3; e += kW) S.cbin[e] = calc_bin(c, S.tim[e], false, status);
+    for (int e = N + lane; e < N + 3; e += kW) {
+        Leaf lf = L.leaf[e];
+        lf.meta = make_meta(calc_bin(c, lf.tim, false, status), -2);
+        L.leaf[e] = lf;
+    }
     status = wave_or(status);
     if (status && !c.status) c.status = status;
+    if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
-    rescan_all(S, L, N + 3);
+    rescan_all(L, N + 3);
 }
 
 __device__ void rebuild(const SysDev& S, const DevTables& T, const Lds& L, Sc& c) {
     const int lane = threadIdx.x;
     for (int k = lane; k < S.N; k += kW) {
-        Bead b = S.bead[k];
+        Rec b = S.rec[k];
         b.x = b.x - round(b.x);
         b.y = b.y - round(b.y);
         b.z = b.z - round(b.z);
-        S.bead[k] = b;
+        S.rec[k] = b;
         S.old_r[k] = make_double4(b.x, b.y, b.z, 0.0);
     }
+    __threadfence_block();
     __syncthreads();
-    rebuild_lists(S, T, L, c);
+    rebuild_lists(S, T, c);
     events_all(S, T, L, c);
 }
 
@@ -1325,16 +1519,16 @@ __device__ void sync_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
     const int N = S.N;
     const double tf = c.tfalse;
     c.t = c.t + tf;
-    for (int e = lane; e < N + 3; e += kW) S.tim[e] = S.tim[e] - tf;
+    for (int e = lane; e < N + 3; e += kW) L.leaf[e].tim = L.leaf[e].tim - tf;
     c.interval_max = c.interval_max - tf;
     c.tbin_off = c.tbin_off + tf;
     double moved_far = 0.0;
     for (int k = lane; k < N; k += kW) {
-        Bead b = S.bead[k];
+        Rec b = S.rec[k];
         b.x = b.x + b.vx * tf;
         b.y = b.y + b.vy * tf;
         b.z = b.z + b.vz * tf;
-        S.bead[k] = b;
+        S.rec[k] = b;
         const double4 o = S.old_r[k];
         const double a = o.x - b.x, bb = o.y - b.y, cc = o.z - b.z;
         const double dis = a * a + bb * bb + cc * cc;
@@ -1351,6 +1545,7 @@ __device__ void sync_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
             c.interval = c.t_fact / sqrt(S.setemp);
         }
     }
+    __threadfence_block();
     __syncthreads();
     if (update || c.interval > c.interval_max) {
         if (!update) {
@@ -1363,18 +1558,22 @@ __device__ void sync_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
         c.nupdates++;
         rebuild(S, T, L, c);
     } else {
-        rescan_all(S, L, N + 3);
+        rescan_all(L, N + 3);
     }
     int status = 0;
     const double t2 = c.interval * 0.999;
     if (lane == 0) {
-        S.tim[N + 1] = t2;
-        S.cbin[N + 1] = calc_bin(c, t2, true, status);
+        Leaf lf;
+        lf.tim = t2;
+        lf.nptnr = -2;
+        lf.meta = make_meta(calc_bin(c, t2, true, status), -2);
+        L.leaf[N + 1] = lf;
         L.blist[0] = (N + 1) / kBlk;
     }
     if (status && !c.status) c.status = status;
+    if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
-    rescan_blocks(S, L, N + 3, 1);
+    rescan_block_list(L, N + 3, 1);
 }
 
 __device__ void output_event(const SysDev& S, const Lds& L, Sc& c) {
@@ -1383,35 +1582,46 @@ __device__ void output_event(const SysDev& S, const Lds& L, Sc& c) {
     int status = 0;
     const double t3 = 3.3 / (sqrt(S.setemp)) + 5 + c.tfalse;
     if (lane == 0) {
-        S.tim[N + 2] = t3;
-        S.cbin[N + 2] = calc_bin(c, t3, false, status);
+        Leaf lf;
+        lf.tim = t3;
+        lf.nptnr = -2;
+        lf.meta = make_meta(calc_bin(c, t3, false, status), -2);
+        L.leaf[N + 2] = lf;
         L.blist[0] = (N + 2) / kBlk;
     }
     if (status && !c.status) c.status = status;
+    if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
-    rescan_blocks(S, L, N + 3, 1);
+    rescan_block_list(L, N + 3, 1);
 }
 
-__device__ Lds make_lds(int nblk, int maxd) {
+__device__ Lds make_lds(const SysDev& S, int nblk_max, int maxw, int leaf_cap) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     Lds L;
-    L.nblk = nblk;
-    L.maxd = maxd;
-    L.maxblist = 256;
+    const int N3 = S.N + 3;
+    L.nblk = (N3 + kBlk - 1) / kBlk;
+    L.nsup = (L.nblk + kSup - 1) / kSup;
+    L.maxw = maxw;
     unsigned char* p = smem;
-    L.bt_t = (double*)p;
-    p += sizeof(double) * nblk;
-    L.bt_b = (int*)p;
-    p += sizeof(int) * nblk;
-    L.bt_i = (int*)p;
-    p += sizeof(int) * nblk;
-    L.dl_l = (int*)p;
-    p += sizeof(int) * maxd;
-    L.dl_tag = (int*)p;
-    p += sizeof(int) * maxd;
+    L.tab = (DevTables*)p;
+    p += (sizeof(DevTables) + 15) / 16 * 16;
+    L.blk = (BEnt*)p;
+    p += sizeof(BEnt) * nblk_max;
+    L.sup = (BEnt*)p;
+    p += sizeof(BEnt) * kSup;
+    L.wm = (int*)p;
+    p += sizeof(int) * maxw;
+    L.wx = (int*)p;
+    p += sizeof(int) * maxw;
     L.blist = (int*)p;
-    p += sizeof(int) * L.maxblist;
-    L.misc = (int*)p;
+    p += sizeof(int) * kMaxBlist;
+    p = (unsigned char*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+    L.pair = (Pair*)p;
+    p += (sizeof(Pair) + 15) / 16 * 16;
+    L.sc = (Sc*)p;
+    p += (sizeof(Sc) + 15) / 16 * 16;
+    L.leaf_lds = N3 <= leaf_cap;
+    L.leaf = L.leaf_lds ? (Leaf*)p : S.leaf;
     return L;
 }
 
@@ -1449,10 +1659,17 @@ __device__ inline void store_sc(SysState* st, const Sc& c) {
     st->numghosts = c.numghosts;
     st->nupdates = c.nupdates;
     st->nforced = c.nforced;
+    st->target = c.target;
     st->rng_ctr[0] = c.rc0;
     st->rng_ctr[1] = c.rc1;
     st->nbin = c.nbin;
     st->status = c.status;
+}
+__device__ void copy_tables(const DevTables* src, DevTables* dst) {
+    const double* s = (const double*)src;
+    double* d = (double*)dst;
+    const int n = (int)(sizeof(DevTables) / sizeof(double));
+    for (int k = threadIdx.x; k < n; k += kW) d[k] = s[k];
 }
 
 }  // namespace
@@ -1460,85 +1677,81 @@ __device__ inline void store_sc(SysState* st, const Sc& c) {
 // ------------------------------------------------------------------------
 // kernels
 // ------------------------------------------------------------------------
-// Set-up of one system per wavefront: wrap, lists, pseudo-event seeds and
+// Start-up of one system per wavefront: wrap, lists, pseudo-event seeds and
 // events() (main.F90:205-424 after the host-side H-bond restore).
-__global__ __launch_bounds__(64) void k_setup(const SysDev* sys, const int* ids, int nblk, int maxd) {
+__global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, const int* ids, int nblk_max, int maxw) {
     const SysDev& S = sys[ids[blockIdx.x]];
-    const DevTables& T = *S.tab;
-    Lds L = make_lds(nblk, maxd);
-    Sc c;
-    load_sc(S.st, c);
+    Lds L = make_lds(S, nblk_max, maxw, 0);
+    copy_tables(S.tab, L.tab);
+    __syncthreads();
+    const DevTables& T = *L.tab;
+    Sc& c = *L.sc;
+    if (threadIdx.x == 0) load_sc(S.st, c);
+    __syncthreads();
     const int N = S.N;
-    for (int k = threadIdx.x; k < N; k += kW) {
-        S.tim[k] = c.interval_max + kLt;
-        S.coltype[k] = -1;
-        S.nptnr[k] = -1;
-    }
     if (threadIdx.x == 0) {
+        Leaf g, s2, s3;
         if (S.canon) {
             double tgho = 0.0;
             int guard = 0;
             do {
                 tgho = rng_u01(S.seed, 0, c.rc0++);
             } while ((tgho < 1.e-18 || tgho == 1.0) && ++guard < 1000);
-            S.tim[N] = -1.0 * dlog(tgho) * S.avegtime * .0000001;
+            g.tim = -1.0 * dlog(tgho) * S.avegtime * .0000001;
         } else
-            S.tim[N] = 1000000000.0;
-        S.tim[N + 1] = c.interval;
-        S.tim[N + 2] = 3.3 / (sqrt(S.setemp)) + 5;
-        for (int e = N; e < N + 3; ++e) {
-            S.nptnr[e] = -2;
-            S.coltype[e] = -2;
-        }
+            g.tim = 1000000000.0;
+        s2.tim = c.interval;
+        s3.tim = 3.3 / (sqrt(S.setemp)) + 5;
+        g.nptnr = s2.nptnr = s3.nptnr = -2;
+        g.meta = s2.meta = s3.meta = make_meta(-1, -2);
+        S.leaf[N] = g;
+        S.leaf[N + 1] = s2;
+        S.leaf[N + 2] = s3;
     }
-    c.rc0 = __shfl(c.rc0, 0, kW);
+    __threadfence_block();
     __syncthreads();
     rebuild(S, T, L, c);
     if (threadIdx.x == 0) store_sc(S.st, c);
 }
 
 // The event loop: each wavefront commits events of its system until the
-// system's coll counter reaches its target.
-__global__ __launch_bounds__(64) void k_run(const SysDev* sys, const int* ids, const long long* inc, int nblk,
-                                           int maxd) {
+// system's coll counter has advanced by inc[k].
+__global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc, int nblk_max,
+                                            int maxw, int leaf_cap) {
     const SysDev& S = sys[ids[blockIdx.x]];
-    const DevTables& T = *S.tab;
-    Lds L = make_lds(nblk, maxd);
-    Sc c;
-    load_sc(S.st, c);
-    c.target = c.coll + inc[blockIdx.x];
-    const int N = S.N, N3 = N + 3;
-    rescan_all(S, L, N3);
+    Lds L = make_lds(S, nblk_max, maxw, leaf_cap);
     const int lane = threadIdx.x;
+    const int N = S.N, N3 = N + 3;
+    copy_tables(S.tab, L.tab);
+    if (L.leaf_lds)
+        for (int e = lane; e < N3; e += kW) L.leaf[e] = S.leaf[e];
+    __syncthreads();
+    const DevTables& T = *L.tab;
+    Sc& c = *L.sc;
+    if (lane == 0) {
+        load_sc(S.st, c);
+        c.target = c.coll + inc[blockIdx.x];
+    }
+    __syncthreads();
+    rescan_all(L, N3);
     int log_count = S.st->log_count;
     while (c.coll < c.target && c.status == 0) {
-        // next event: min over the block table of (bin, time, index)
-        int kb = INT_MAX, ki = INT_MAX;
-        double kt = 0.0;
-        for (int b = lane; b < L.nblk; b += kW) {
-            const int ob = L.bt_b[b];
-            const double ot = L.bt_t[b];
-            const int oi = L.bt_i[b];
-            if (key_less(ob, ot, oi, kb, kt, ki)) {
-                kb = ob;
-                kt = ot;
-                ki = oi;
-            }
-        }
-        wave_min_key(kb, kt, ki, kW);
-        if (kb == INT_MAX) {
+        // next event: min over the superblock minima of (bin, time, index)
+        BEnt k = lane < L.nsup ? L.sup[lane] : empty_ent();
+        red_key(k, kW);
+        if (k.bin == INT_MAX) {
             c.status = kErrBin;
             break;
         }
-        const int e = ki;
+        const int e = k.idx;
         c.old_tfalse = c.tfalse;
-        c.nbin = kb;
-        c.tfalse = kt;
+        c.nbin = k.bin;
+        c.tfalse = k.t;
         c.coll++;
         int jj = -1, ty = 0;
         if (e < N) {
-            jj = S.nptnr[e];
-            ty = pair_event(S, T, L, c, e);
+            jj = k.p;
+            ty = pair_event(S, T, L, c, e, k.p, k.ct);
             if (ty >= 1 && ty <= 30 && lane == 0) S.st->nevents[ty]++;
         } else if (e == N) {
             jj = ghost_event(S, T, L, c);
@@ -1555,12 +1768,15 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* sys, const int* ids, c
                 r.j = jj + 1;
                 r.type = ty;
                 r.pad = 0;
-                r.t = kt;
+                r.t = k.t;
                 S.log[log_count] = r;
             }
             ++log_count;
         }
     }
+    __syncthreads();
+    if (L.leaf_lds)
+        for (int e2 = lane; e2 < N3; e2 += kW) S.leaf[e2] = L.leaf[e2];
     if (lane == 0) {
         S.st->log_count = log_count;
         store_sc(S.st, c);
@@ -1579,35 +1795,49 @@ __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const
         type[k] = -100;
         return;
     }
-    const int code = dyn_code(S, j, i, static_code(S, j, i));
-    double t;
-    int ty;
-    predict_pair(S, *S.tab, i, j, code, S.st->tfalse, t, ty);
+    const Rec ri = S.rec[i], rj = S.rec[j];
+    const int code = dyn_code(S.ovr[j], i, static_t(rj.topo, ri.topo));
+    double t = 1000000000.0;
+    int ty = 0;
+    predict(*S.tab, code, ri, rj, dyn_bptnr(ri.dyn) == j, S.bw[i], S.st->tfalse, t, ty);
     tij[k] = t;
     type[k] = ty;
 }
 
-size_t engine_lds_bytes(int nblk, int maxd) {
-    return sizeof(double) * nblk + 2 * sizeof(int) * nblk + 2 * sizeof(int) * maxd + sizeof(int) * 256 + 64;
+size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap) {
+    size_t b = (sizeof(DevTables) + 15) / 16 * 16 + sizeof(BEnt) * nblk_max + sizeof(BEnt) * kSup +
+               2 * sizeof(int) * maxw + sizeof(int) * kMaxBlist + 16 + (sizeof(Pair) + 15) / 16 * 16 +
+               (sizeof(Sc) + 15) / 16 * 16;
+    b += sizeof(Leaf) * (size_t)leaf_cap;
+    return b;
 }
+static int nblk_for(int n) { return (n + 3 + kBlk - 1) / kBlk; }
 
-hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, hipStream_t s) {
-    const int nblk = (max_n + 3 + kBlk - 1) / kBlk;
-    const size_t lds = engine_lds_bytes(nblk, maxd);
-    hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW), lds, s, d_sys, d_ids, nblk, maxd);
+hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxw, hipStream_t s) {
+    const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0);
+    hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW), lds, s, d_sys, d_ids, nblk_for(max_n), maxw);
     return hipGetLastError();
 }
-hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxd,
-                      hipStream_t s) {
-    const int nblk = (max_n + 3 + kBlk - 1) / kBlk;
-    const size_t lds = engine_lds_bytes(nblk, maxd);
-    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxd);
+hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
+                      size_t lds_limit, hipStream_t s) {
+    const int nblk = nblk_for(max_n);
+    const size_t base = engine_lds_bytes(nblk, maxw, 0);
+    int leaf_cap = 0;
+    if (lds_limit > base) leaf_cap = (int)((lds_limit - base) / sizeof(Leaf));
+    if (leaf_cap > max_n + 3) leaf_cap = max_n + 3;
+    const size_t lds = engine_lds_bytes(nblk, maxw, leaf_cap);
+    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap);
     return hipGetLastError();
 }
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_predict, dim3((n + 255) / 256), dim3(256), 0, s, d_sys, id, n, pi, pj, tij, type);
     return hipGetLastError();
+}
+hipError_t engine_prepare(size_t lds_limit) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
 }
 
 }  // namespace pdmd
