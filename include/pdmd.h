@@ -124,6 +124,12 @@ int pdmd_log_read(pdmd_ctx* ctx, int32_t sys_id, int32_t max, pdmd_event_t* out,
 int pdmd_predict_pairs(pdmd_ctx* ctx, int32_t sys_id, int32_t n, const int32_t* i, const int32_t* j, double* tij,
                        int32_t* type);
 
+/* Pair codes ev_code(a,b) as make_code.f:18-566 builds them for the system
+ * described (before any H-bond restore), for n 1-based bead pairs; host
+ * only, needs no device.  The device derives the same codes from compact
+ * per-bead topology words instead of the N x N int8 matrix. */
+int pdmd_pair_codes(const pdmd_system_desc* desc, int32_t n, const int32_t* a, const int32_t* b, int32_t* code);
+
 /* Observables of the output pseudo-event (energy.f, radgyr.f, end2end.f). */
 int pdmd_observables(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
 
@@ -134,6 +140,13 @@ int pdmd_observables(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
  * restype/hp hold sum(chain_len) entries, sv 6*N. */
 int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_len, double density, double tstar,
                   uint64_t seed, int32_t* restype, int32_t* hp, double* sv, double* boxl);
+/* The same for a given sequence: restype_in holds sum(chain_len) side-chain
+ * identities (9..28), copied to restype; NULL draws it as pdmd_generate does.
+ * Chains that share a sequence can form one species (the reference's
+ * nop1/nop2 blocks, inputinfo.f:230-262). */
+int pdmd_generate_seq(const char* param_dir, int32_t nchains, const int32_t* chain_len, const int32_t* restype_in,
+                      double density, double tstar, uint64_t seed, int32_t* restype, int32_t* hp, double* sv,
+                      double* boxl);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,10 @@ def lib():
         L.orc_create.argtypes = [ctypes.c_char_p, ctypes.c_int, _i32p, _i32p, _i32p, _i32p, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_uint64, ctypes.c_int, _f64p, _i32p, ctypes.c_char_p,
                                  ctypes.c_int]
+        L.orc_create_ex.restype = ctypes.c_void_p
+        L.orc_create_ex.argtypes = [ctypes.c_char_p, ctypes.c_int, _i32p, _i32p, _i32p, _i32p, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _f64p, _i32p,
+                                    ctypes.c_char_p, ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_run.restype = ctypes.c_int64
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64]
@@ -84,7 +88,7 @@ def _p(a, t):
 class Oracle:
     """One system in the CPU oracle."""
 
-    def __init__(self, system, seed, param_dir, canon=True, bptnr=None, species=None):
+    def __init__(self, system, seed, param_dir, canon=True, bptnr=None, species=None, no_hbs=False):
         self.cl = np.ascontiguousarray(system["chain_len"], np.int32)
         self.rt = np.ascontiguousarray(system["restype"], np.int32)
         self.hp = np.ascontiguousarray(system["hp"], np.int32)
@@ -93,11 +97,11 @@ class Oracle:
         N = 4 * int(self.cl.sum())
         self.bp = np.zeros(N, np.int32) if bptnr is None else np.ascontiguousarray(bptnr, np.int32)
         err = ctypes.create_string_buffer(256)
-        self.h = lib().orc_create(param_dir.encode(), len(self.cl), _p(self.cl, ctypes.c_int32),
-                                  _p(self.sp, ctypes.c_int32), _p(self.rt, ctypes.c_int32),
-                                  _p(self.hp, ctypes.c_int32), float(system["boxl"]), float(system["tstar"]),
-                                  int(seed), 1 if canon else 0, _p(self.sv, ctypes.c_double),
-                                  _p(self.bp, ctypes.c_int32), err, 256)
+        self.h = lib().orc_create_ex(param_dir.encode(), len(self.cl), _p(self.cl, ctypes.c_int32),
+                                     _p(self.sp, ctypes.c_int32), _p(self.rt, ctypes.c_int32),
+                                     _p(self.hp, ctypes.c_int32), float(system["boxl"]), float(system["tstar"]),
+                                     int(seed), 1 if canon else 0, 1 if no_hbs else 0, _p(self.sv, ctypes.c_double),
+                                     _p(self.bp, ctypes.c_int32), err, 256)
         if not self.h:
             raise RuntimeError("orc_create failed: " + err.value.decode())
         self.N = N

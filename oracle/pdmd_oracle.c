@@ -15,15 +15,25 @@
  * Compile with -ffp-contract=off: floating-point expressions below keep the
  * reference's evaluation order (Fortran left-to-right for equal precedence).
  *
- * Parity status per function (see tests/):
- *   params/scale_down/make_code/nbor_setup  -> pinned against the reference
- *       subroutines compiled under oracle/ref (and golden fixtures)
- *   predictors, eventdyn, bumpoff, repuls_* -> pinned the same way plus
- *       closed-form invariants
- *   nbor/cell lists, events, partial_events, calendar -> pinned against the
- *       reference subroutines driven by oracle/ref's serial harness
- *   ghost RNG stream -> project-defined (IFPORT drandm absent): parity
- *       against the reference unpinned by construction; pinned by KATs.
+ * Parity status per function:
+ *   pinned bit for bit against the reference's own subroutines (compiled by
+ *   oracle/ref from the unmodified sources; fixtures in tests/golden/):
+ *     load_params + scale_down (every table), make_code (ev_code, also with
+ *     -Dno_hbs), nbor_setup/cell_link/nbor (lists, in the reference's
+ *     order), events (calendar), core/bond/sqwel/sqshlder/nc_sqwel (every
+ *     pair), eventdyn for coltypes 1-3 and side-chain wells 4/8,
+ *     partial_events/eventredo_up/eventredo_down/add_tbin/del_tbin, displ
+ *     and the sync and output pseudo-events over 1.8e5-2e5 calendar entries,
+ *     energy/radgyr/end_to_end;
+ *   pinned by invariants only (NVE energy conservation to 1e-10, calendar
+ *   equal to a brute-force minimum, list membership by brute force):
+ *     the worker's resolution of N-C well events (coltypes 7/10/12 ->
+ *     4/5/6/9/13/14/15/16; repuls_check, repuls_check_3, check_sigma,
+ *     repuls_add, repuls_del_a/b, bumpoff, H-bond bookkeeping) -- parity
+ *     against the reference unpinned for these;
+ *   ghost and terminal-capture random draws: project-defined RNG in place
+ *     of IFPORT drandm (absent), pinned by the splitmix64 KAT; parity
+ *     against the reference unpinned by construction.
  */
 #include "pdmd_oracle.h"
 
@@ -95,6 +105,7 @@ struct orc_sys {
     double tfalse, t, interval, interval_max, t_fact, n_forced, sortsize, tbin_off;
     double setemp, avegtime, sig_max_all, hdelr, rlsq[51];
     int canon;
+    int no_hbs; /* -Dno_hbs: no N-C hydrogen-bond wells (make_code.f:114-170,192-208,348-365) */
     int64_t coll, numghosts, nupdates, nforced, nevents[31];
     uint64_t seed, rng_ctr[2];
     double old_tfalse;
@@ -354,24 +365,26 @@ static void make_code(orc_sys* S) {
             if (id[m] > 8 && id[l] > 8 && S->hp_bead[m] == 1 && S->hp_bead[l] == 1 &&
                 S->chnnum[l] != S->chnnum[m])
                 EV(S, m, l) = 16;
-    /* all nc's: 15 (:108-117) */
-    for (int m = 1; m <= N - 1; ++m)
-        for (int l = m + 1; l <= N; ++l)
-            if ((id[m] == 1 && id[l] == 4) || (id[m] == 4 && id[l] == 1)) EV(S, m, l) = 15;
-    /* same-chain reset (:120-136) */
-    for (int c = 0; c < S->nchains; ++c) {
-        int kk = S->chain_start[c], nbc = 4 * S->chain_len[c];
-        for (int m = kk + 1; m <= kk + nbc - 1; ++m)
-            for (int l = m + 1; l <= kk + nbc; ++l) EV(S, m, l) = 1;
-    }
-    /* proline NH rule (:139-161) */
-    for (int m = 1; m <= N - 1; ++m)
-        for (int l = m + 1; l <= N; ++l) {
-            if (id[m] == 17 && id[l] == 4)
-                EV(S, m - 2 * S->cl[m], l) = 1;
-            else if (id[m] == 4 && id[l] == 17)
-                EV(S, l - 2 * S->cl[l], m) = 1;
+    if (!S->no_hbs) {
+        /* all nc's: 15 (:108-117) */
+        for (int m = 1; m <= N - 1; ++m)
+            for (int l = m + 1; l <= N; ++l)
+                if ((id[m] == 1 && id[l] == 4) || (id[m] == 4 && id[l] == 1)) EV(S, m, l) = 15;
+        /* same-chain reset (:120-136) */
+        for (int c = 0; c < S->nchains; ++c) {
+            int kk = S->chain_start[c], nbc = 4 * S->chain_len[c];
+            for (int m = kk + 1; m <= kk + nbc - 1; ++m)
+                for (int l = m + 1; l <= kk + nbc; ++l) EV(S, m, l) = 1;
         }
+        /* proline NH rule (:139-161) */
+        for (int m = 1; m <= N - 1; ++m)
+            for (int l = m + 1; l <= N; ++l) {
+                if (id[m] == 17 && id[l] == 4)
+                    EV(S, m - 2 * S->cl[m], l) = 1;
+                else if (id[m] == 4 && id[l] == 17)
+                    EV(S, l - 2 * S->cl[l], m) = 1;
+            }
+    }
     /* per chain block (:167-329 species 1, :331-484 species 2) */
     for (int c = 0; c < S->nchains; ++c) {
         int lll = S->chain_start[c], L = S->chain_len[c];
@@ -383,11 +396,13 @@ static void make_code(orc_sys* S) {
                     EV(S, lll + FS(k), lll + FS(l)) = 16;
             ncount = ncount + 1;
         }
-        for (int k = lll + L + 1; k <= lll + 2 * L; ++k)
-            for (int l = lll + 2 * L + 1; l <= lll + 3 * L; ++l) EV(S, k, l) = 15;
-        for (int k = lll + L + 1; k <= lll + 2 * L; ++k)
-            for (int l = k + L - N_B_HBOND; l <= k + L + N_B_HBOND; ++l)
-                if (l >= 1 && l <= N) EV(S, k, l) = 1;
+        if (!S->no_hbs) {
+            for (int k = lll + L + 1; k <= lll + 2 * L; ++k)
+                for (int l = lll + 2 * L + 1; l <= lll + 3 * L; ++l) EV(S, k, l) = 15;
+            for (int k = lll + L + 1; k <= lll + 2 * L; ++k)
+                for (int l = k + L - N_B_HBOND; l <= k + L + N_B_HBOND; ++l)
+                    if (l >= 1 && l <= N) EV(S, k, l) = 1;
+        }
         for (int k = lll + 1; k <= lll + L; ++k) EV(S, k, L + k) = 4;
         for (int k = lll + 1; k <= lll + L; ++k) EV(S, k, 2 * L + k) = 5;
         for (int k = lll + L + 2; k <= lll + 2 * L; ++k) EV(S, k, k + L - 1) = 6;
@@ -1651,8 +1666,18 @@ void orc_observables(const orc_sys* S, orc_obs* o) {
     double* st = (double*)malloc(sizeof(double) * 3 * (size_t)(N + 1));
     double rg = 0.0, e2e = 0.0;
     int nbb = 0;
+    /* chains of the leading species block are the reference's species 1 */
+    int nfirst = 0;
+    while (nfirst < S->nchains && S->species[nfirst] == S->species[0]) nfirst++;
     for (int c = 0; c < S->nchains; ++c) {
         int ii = S->chain_start[c], L = S->chain_len[c];
+        /* radgyr.f:117-125 sums a species-2 chain i over beads
+         * (i-1)*numbeads2+1 .. (i-1)*numbeads2+3*chnln2, without the nop1
+         * offset its unwrapping uses (:78); kept as the reference computes
+         * it while those beads are already filled, the chain's own beads
+         * otherwise (the reference would read unset memory there) */
+        int rb = ii;
+        if (c >= nfirst && c * 4 * L + 3 * L <= ii + 4 * L) rb = c * 4 * L;
         for (int jb = ii + 1; jb <= ii + 4 * L; ++jb)
             for (int k = 1; k <= 3; ++k) st[3 * jb + k - 1] = SV(S, k, jb) + SV(S, k + 3, jb) * tf;
 #define UNWRAP(a, b)                                               \
@@ -1668,7 +1693,7 @@ void orc_observables(const orc_sys* S, orc_obs* o) {
         for (int jb = ii + 1; jb <= ii + L; ++jb) UNWRAP(2 * L + jb, jb);
 #undef UNWRAP
         double cx = 0.0, cy = 0.0, cz = 0.0, rgsum = 0.0;
-        for (int jb = ii + 1; jb <= ii + 3 * L; ++jb) {
+        for (int jb = rb + 1; jb <= rb + 3 * L; ++jb) {
             cx = cx + st[3 * jb];
             cy = cy + st[3 * jb + 1];
             cz = cz + st[3 * jb + 2];
@@ -1676,7 +1701,7 @@ void orc_observables(const orc_sys* S, orc_obs* o) {
         cx = cx / (double)(3 * L);
         cy = cy / (double)(3 * L);
         cz = cz / (double)(3 * L);
-        for (int jb = ii + 1; jb <= ii + 3 * L; ++jb) {
+        for (int jb = rb + 1; jb <= rb + 3 * L; ++jb) {
             double rx = (st[3 * jb] - cx) * (st[3 * jb] - cx);
             double ry = (st[3 * jb + 1] - cy) * (st[3 * jb + 1] - cy);
             double rz = (st[3 * jb + 2] - cz) * (st[3 * jb + 2] - cz);
@@ -1736,6 +1761,14 @@ orc_sys* orc_create(const char* param_dir, int nchains, const int32_t* chain_len
                     const int32_t* species, const int32_t* restype, const int32_t* hp,
                     double boxl_angstrom, double tstar, uint64_t seed, int canon,
                     const double* sv, const int32_t* bptnr, char* err, int errlen) {
+    return orc_create_ex(param_dir, nchains, chain_len, species, restype, hp, boxl_angstrom, tstar, seed, canon, 0,
+                         sv, bptnr, err, errlen);
+}
+
+orc_sys* orc_create_ex(const char* param_dir, int nchains, const int32_t* chain_len,
+                       const int32_t* species, const int32_t* restype, const int32_t* hp,
+                       double boxl_angstrom, double tstar, uint64_t seed, int canon, int flags,
+                       const double* sv, const int32_t* bptnr, char* err, int errlen) {
     orc_sys* S = (orc_sys*)calloc(1, sizeof(orc_sys));
     char dummy[8];
     if (!err) {
@@ -1761,6 +1794,7 @@ orc_sys* orc_create(const char* param_dir, int nchains, const int32_t* chain_len
     }
     S->N = N;
     S->canon = canon;
+    S->no_hbs = (flags & ORC_NO_HBS) != 0;
     S->seed = seed;
     size_t n1 = (size_t)N + 4;
     S->chnnum = (int*)calloc(n1, sizeof(int));

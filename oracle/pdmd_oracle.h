@@ -55,6 +55,13 @@ orc_sys* orc_create(const char* param_dir, int nchains, const int32_t* chain_len
                     const int32_t* species, const int32_t* restype, const int32_t* hp,
                     double boxl_angstrom, double tstar, uint64_t seed, int canon,
                     const double* sv, const int32_t* bptnr, char* err, int errlen);
+/* The same with build flags: ORC_NO_HBS restates -Dno_hbs (no N-C
+ * hydrogen-bond wells, make_code.f:114-170,192-208,348-365). */
+#define ORC_NO_HBS 1
+orc_sys* orc_create_ex(const char* param_dir, int nchains, const int32_t* chain_len,
+                       const int32_t* species, const int32_t* restype, const int32_t* hp,
+                       double boxl_angstrom, double tstar, uint64_t seed, int canon, int flags,
+                       const double* sv, const int32_t* bptnr, char* err, int errlen);
 void orc_destroy(orc_sys* s);
 
 int32_t orc_nbeads(const orc_sys* s);

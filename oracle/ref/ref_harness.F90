@@ -1,0 +1,322 @@
+! ref_harness.F90 -- test infrastructure: drives the reference's own
+! subroutines (compiled from their unmodified sources, pulled in below with
+! #include through symlinks that oracle/ref/Makefile makes) to produce
+! golden vectors for the CPU oracle.  Nothing here is shipped or measured.
+!
+! stdin: line 1 T* (as temp_XXX line 1, main.F90:124-127), line 2 the number
+! of pair events of the serial loop below.  Reads the reference's input files
+! from the working directory (parameters/, parametersep/, helix.inp start via
+! -Drunh, inputinfo.f:28-66) and writes raw little-endian stream files:
+!   setup.bin  scaled tables, bond tables, masses, cell geometry, sv
+!   evcode.bin ev_code(N,N) as int32, column-major
+!   nb.bin     na_npt(N), nnabdn(N), then each bead's up list and down list
+!   cal0.bin   tim(N+3), nptnr(N+3), coltype(N+3) after events()
+!   pred.bin   for every pair i<j (row-major in i): tij, type as events.f
+!              dispatches them (events.f:26-47)
+!   loop.bin   per committed entry (i, j, resolved coltype, tfalse)
+!   loopn.bin  entries, pair events, neighbour-list rebuilds, forced rebuilds
+!   final.bin  sv, tim, nptnr, coltype after the loop
+!   obs.bin    energy/radgyr/end_to_end before and after the loop
+!
+! The serial loop commits calendar entries in the reference's order: bead
+! pairs whose outcome needs no resolution by the master/worker code of
+! main.F90 (coltype 1-3, and 4/8 between two side chains), and the sync and
+! output pseudo-events.  It stops at the first pair event outside that set.
+
+#include "def.h"
+#include "header.f"
+
+	program ref_harness
+	use global
+	implicit none
+	integer i, j, k, kk, m, evcode, typ, nsteps, npair, nbrsum, nlog, nupdates_h, nforced_h
+	logical update
+	real*8 tim_sel
+	integer, allocatable :: code32(:)
+	integer, allocatable :: coltypes(:)
+	real*8 tstar_in, tij, tmin, ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg
+	integer hb_alpha, hb_ii, hb_ij
+	logical direct
+	real*8 ev_rec(2)
+	integer ev_int(3)
+
+	noptotal = nop1 + nop2
+	coll = 0
+	nupdates_h = 0
+	nforced_h = 0
+	update = .false.
+	fname_digits_pre = '0000'
+	read(5,*) tstar_in
+	read(5,*) nsteps
+	setemp = tstar_in*12.d0
+	pi = 4.d0*datan(1.d0)
+	! main.F90:144-149
+	t_fact = 0.00005D0
+	n_forced = 150.0d0
+	interval = t_fact/dsqrt(setemp)
+	interval_max = n_forced*interval
+	sortsize = interval_max/dble(numbin)
+
+	call inputinfo()
+	call make_code()
+
+	! main.F90:196-230
+	t = 0.d0
+	tfalse = 0.d0
+	nbin = 1
+	tbin_off = 0.d0
+	do k = 1, noptotal
+		sv(1,k) = sv(1,k) - dnint(sv(1,k))
+		sv(2,k) = sv(2,k) - dnint(sv(2,k))
+		sv(3,k) = sv(3,k) - dnint(sv(3,k))
+		old_rx(k) = sv(1,k)
+		old_ry(k) = sv(2,k)
+		old_rz(k) = sv(3,k)
+		tim(k) = interval_max + ltstep
+		coltype(k) = -1
+		nptnr(k) = -1
+		npt(k) = (k-1)*maxnbs + 1
+		npt_dn(k) = (k-1)*maxnbs + 1
+		bptnr(k) = 0
+		do kk = 1, 4
+			extra_repuls(k,kk) = 0
+		end do
+	end do
+	npt(noptotal+1) = noptotal*maxnbs + 1
+	npt_dn(noptotal+1) = noptotal*maxnbs + 1
+	do k = noptotal+1, noptotal+3
+		nptnr(k) = -2
+		coltype(k) = -2
+	end do
+	do k = 0, numbin+1
+		bin(k) = 0
+	end do
+	do k = 1, noptotal+3
+		tlinks(k) = 0
+		tlinks2(k) = 0
+	end do
+
+	! main.F90:389-424 (no Andersen thermostat: tim(N+1) = 1e9)
+	call nbor_setup()
+	num_cell = int(boxl/(sig_max_all*rl_const)*n_wrap) + 2*n_wrap
+	allocate(cell(num_cell**3+1))
+	allocate(wrap_map(num_cell**3))
+	width = boxl/dble(num_cell-2*n_wrap)
+	half = boxl/2.d0
+	call cell_link
+	call nbor()
+	nbrsum = 0
+	do k = 1, noptotal-1
+		nbrsum = nbrsum + na_npt(k)
+	end do
+	tim(noptotal+1) = 1000000000.d0
+	tim(noptotal+2) = interval
+	tim(noptotal+3) = 3.3/(dsqrt(setemp)) + 5
+	call events()
+
+	open(unit=21, file='setup.bin', access='stream', form='unformatted', status='replace')
+	write(21) sigma_sq, welldia_sq, ep_sqrt, shlddia_sq, ev_param, sqz610
+	write(21) bm(1:noptotal)
+	write(21) bdln, bl_rn, bl_rc, del_bdln, del_blrn, del_blrc
+	write(21) rlsq, sig_max_all, hdelr, width, dble(num_cell), interval, interval_max, sortsize, setemp
+	write(21) sv
+	write(21) identity, chnnum, hp
+	close(21)
+
+	allocate(code32(noptotal*noptotal))
+	do j = 1, noptotal
+		do i = 1, noptotal
+			code32((j-1)*noptotal + i) = ev_code(i,j)
+		end do
+	end do
+	open(unit=21, file='evcode.bin', access='stream', form='unformatted', status='replace')
+	write(21) code32
+	close(21)
+
+	open(unit=21, file='nb.bin', access='stream', form='unformatted', status='replace')
+	write(21) na_npt, nnabdn
+	do i = 1, noptotal
+		if (na_npt(i) > 0) write(21) nb(npt(i):npt(i)+na_npt(i)-1)
+	end do
+	do i = 1, noptotal
+		if (nnabdn(i) > 0) write(21) dnnab(npt_dn(i):npt_dn(i)+nnabdn(i)-1)
+	end do
+	close(21)
+
+	open(unit=21, file='cal0.bin', access='stream', form='unformatted', status='replace')
+	write(21) tim, nptnr, coltype
+	close(21)
+
+	! every pair, dispatched on ev_code as events.f:26-47
+	open(unit=21, file='pred.bin', access='stream', form='unformatted', status='replace')
+	allocate(coltypes(noptotal+3))
+	coltypes = coltype
+	do i = 1, noptotal-1
+		do j = i+1, noptotal
+			tij = 1000000000.d0
+			typ = 0
+			evcode = ev_code(j,i)
+			if (evcode .le. 3) then
+				call core(i,j,evcode,tij,typ)
+			elseif ((evcode .ge. 4) .and. (evcode .le. 12)) then
+				call bond(i,j,evcode,tij,typ)
+			elseif (evcode .eq. 15) then
+				call nc_sqwel(i,j,evcode,tij,typ)
+			elseif (evcode .eq. 16) then
+				call sqwel(i,j,evcode,tij,typ)
+			elseif ((evcode .ge. 17) .and. (evcode .le. 26)) then
+				call core(i,j,evcode,tij,typ)
+			elseif (evcode .ge. 40) then
+				call sqshlder(i,j,evcode,tij,typ)
+			endif
+			write(21) tij, typ
+		end do
+	end do
+	close(21)
+	coltype = coltypes
+
+	call energy(ered,tred,sumvel,hb_alpha,hb_ii,hb_ij,ehh_ii,ehh_ij)
+	call radgyr(rg_avg)
+	call end_to_end(e2e_avg)
+	open(unit=22, file='obs.bin', access='stream', form='unformatted', status='replace')
+	write(22) ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg, hb_alpha, hb_ii, hb_ij
+
+	! serial loop: selection as main.F90:496-545 with one worker (earliest
+	! entry of the first non-empty bin, first in list order on ties), bead
+	! pair events committed directly (eventdyn + partial_events, main.F90:
+	! 1636,926,943), the sync pseudo-event as main.F90:1126-1187 and the
+	! output pseudo-event's reschedule as main.F90:1238-1242
+	open(unit=21, file='loop.bin', access='stream', form='unformatted', status='replace')
+	npair = 0
+	nlog = 0
+	do m = 1, nsteps
+		do while (bin(nbin) == 0)
+			nbin = nbin + 1
+		end do
+		i = bin(nbin)
+		k = tlinks(i)
+		do while (k .ne. 0)
+			if (tim(k) < tim(i)) i = k
+			k = tlinks(k)
+		end do
+		j = 0
+		if (i <= noptotal) then
+			j = nptnr(i)
+			typ = coltype(i)
+			direct = (typ >= 1 .and. typ <= 3)
+			if ((typ == 4 .or. typ == 8) .and. identity(i) >= 9 .and. identity(j) >= 9) direct = .true.
+			if (.not. direct) exit
+		else if (i == noptotal+1) then
+			exit
+		endif
+		tim_sel = tim(i)
+		tfalse = tim(i)
+		coll = coll + 1
+		if (i <= noptotal) then
+			evcode = ev_code(i,j)
+			call eventdyn(i,j,evcode)
+			typ = coltype(i)
+			call partial_events(i,j,.false.)
+			npair = npair + 1
+		else if (i == noptotal+2) then
+			typ = 0
+			t = t + tfalse
+			do k = 1, noptotal+3
+				tim(k) = tim(k) - tfalse
+			end do
+			interval_max = interval_max - tfalse
+			tbin_off = tbin_off + tfalse
+			do k = 1, noptotal
+				sv(1,k) = sv(1,k) + sv(4,k)*tfalse
+				sv(2,k) = sv(2,k) + sv(5,k)*tfalse
+				sv(3,k) = sv(3,k) + sv(6,k)*tfalse
+			end do
+			tfalse = 0.d0
+			call displ(update)
+			if ((update) .or. (interval .gt. interval_max)) then
+				if (.not. update) then
+					nforced_h = nforced_h + 1
+					n_forced = n_forced * 1.01d0
+				endif
+				interval_max = interval * n_forced
+				sortsize = interval_max/dble(numbin)
+				tbin_off = 0.d0
+				update = .false.
+				nupdates_h = nupdates_h + 1
+				do k = 1, noptotal
+					sv(1,k) = sv(1,k) - anint(sv(1,k))
+					sv(2,k) = sv(2,k) - anint(sv(2,k))
+					sv(3,k) = sv(3,k) - anint(sv(3,k))
+					old_rx(k) = sv(1,k)
+					old_ry(k) = sv(2,k)
+					old_rz(k) = sv(3,k)
+				end do
+				call nbor()
+				do k = 1, noptotal
+					tim(k) = interval_max + ltstep
+					coltype(k) = -1
+					nptnr(k) = -1
+				end do
+				call events()
+				nbin = 1
+			endif
+			if (tlinks2(i) .ne. 0) call del_tbin(i)
+			tim(i) = interval*0.999d0
+			call add_tbin(i)
+		else
+			typ = 0
+			if (tlinks2(i) .ne. 0) call del_tbin(i)
+			tim(i) = 3.3d0/(sqrt(setemp)) + 5 + tfalse
+			if (tim(i) .lt. interval_max) call add_tbin(i)
+		endif
+		nlog = nlog + 1
+		ev_int(1) = i
+		ev_int(2) = j
+		ev_int(3) = typ
+		ev_rec(1) = tim_sel
+		write(21) ev_int, ev_rec(1)
+	end do
+	close(21)
+	open(unit=21, file='loopn.bin', access='stream', form='unformatted', status='replace')
+	write(21) nlog, npair, nupdates_h, nforced_h
+	close(21)
+
+	open(unit=21, file='final.bin', access='stream', form='unformatted', status='replace')
+	write(21) sv, tim, nptnr, coltype, tfalse
+	close(21)
+
+	call energy(ered,tred,sumvel,hb_alpha,hb_ii,hb_ij,ehh_ii,ehh_ij)
+	call radgyr(rg_avg)
+	call end_to_end(e2e_avg)
+	write(22) ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg, hb_alpha, hb_ii, hb_ij
+	close(22)
+	write(6,'(a,i8,a,f10.4)') 'harness: pair events committed ', npair, ' avg nbors ', dble(nbrsum)/dble(noptotal)
+	end program ref_harness
+
+#include "add_tbin.f"
+#include "del_tbin.f"
+#include "bond.f"
+#include "bumped.f"
+#include "cell_add.f"
+#include "cell_link.f"
+#include "core.f"
+#include "displ.f"
+#include "energy.f"
+#include "eventdyn.f"
+#include "eventredo_up.f"
+#include "eventredo_down.f"
+#include "events.f"
+#include "inputinfo.f"
+#include "make_code.f"
+#include "nbor.f"
+#include "nbor_setup.f"
+#include "nc_sqwel.f"
+#include "partial_events.f"
+#include "repuls_add.f"
+#include "repuls_del_a.f"
+#include "repuls_del_b.f"
+#include "scale_down.f"
+#include "sqshlder.f"
+#include "sqwel.f"
+#include "radgyr.f"
+#include "end2end.f"

@@ -746,11 +746,14 @@ int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
     // radgyr.f:18-135 (backbone beads, chains unwrapped along the backbone)
     double rg = 0.0, e2e = 0.0;
     int nbb = 0;
-    for (size_t c = 0; c < t.chain_len.size(); ++c) {
+    const int nch = (int)t.chain_len.size();
+    int nfirst = 0;  // chains of the leading species block: the reference's species 1
+    while (nfirst < nch && t.species[nfirst] == t.species[0]) nfirst++;
+    std::vector<double> q(3 * (size_t)t.N);
+    for (int c = 0; c < nch; ++c) {
         const int s = t.chain_start[c], L = t.chain_len[c];
-        std::vector<double> q(3 * 4 * (size_t)L);
-        for (int k = 0; k < 4 * L; ++k)
-            for (int d3 = 0; d3 < 3; ++d3) q[3 * k + d3] = pos(s + k, d3);
+        for (int k = s; k < s + 4 * L; ++k)
+            for (int d3 = 0; d3 < 3; ++d3) q[3 * k + d3] = pos(k, d3);
         auto unwrap = [&](int a, int b) {
             for (int d3 = 0; d3 < 3; ++d3) {
                 const double dd = q[3 * a + d3] - q[3 * b + d3];
@@ -758,11 +761,17 @@ int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
                 else if (dd < -1.0 * 0.5) q[3 * a + d3] = q[3 * a + d3] + 1.0;
             }
         };
-        for (int k = 1; k < L; ++k) unwrap(k, k - 1);
-        for (int k = 0; k < L; ++k) unwrap(L + k, k);
-        for (int k = 0; k < L; ++k) unwrap(2 * L + k, k);
+        for (int k = s + 1; k < s + L; ++k) unwrap(k, k - 1);
+        for (int k = s; k < s + L; ++k) unwrap(L + k, k);
+        for (int k = s; k < s + L; ++k) unwrap(2 * L + k, k);
+        // radgyr.f:117-125 sums a species-2 chain i over beads
+        // (i-1)*numbeads2+1 .. (i-1)*numbeads2+3*chnln2, without the nop1
+        // offset its unwrapping uses (:78); reproduced while those beads are
+        // already filled, the chain's own beads otherwise
+        int rb = s;
+        if (c >= nfirst && c * 4 * L + 3 * L <= s + 4 * L) rb = c * 4 * L;
         double cx = 0, cy = 0, cz = 0, rgs = 0;
-        for (int k = 0; k < 3 * L; ++k) {
+        for (int k = rb; k < rb + 3 * L; ++k) {
             cx = cx + q[3 * k];
             cy = cy + q[3 * k + 1];
             cz = cz + q[3 * k + 2];
@@ -770,7 +779,7 @@ int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
         cx = cx / (double)(3 * L);
         cy = cy / (double)(3 * L);
         cz = cz / (double)(3 * L);
-        for (int k = 0; k < 3 * L; ++k) {
+        for (int k = rb; k < rb + 3 * L; ++k) {
             const double rx = (q[3 * k] - cx) * (q[3 * k] - cx), ry = (q[3 * k + 1] - cy) * (q[3 * k + 1] - cy),
                          rz = (q[3 * k + 2] - cz) * (q[3 * k + 2] - cz);
             rgs = rgs + rx + ry + rz;
@@ -788,12 +797,31 @@ int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
         e2e = e2e + std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
     }
     o->rg_avg = std::sqrt(rg / (double)nbb) * P.boxl_orig;
-    o->e2e_avg = e2e / (double)t.chain_len.size() * P.boxl_orig;
+    o->e2e_avg = e2e / (double)nch * P.boxl_orig;
     return 0;
 }
 
-int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_len, double density, double tstar,
-                  uint64_t seed, int32_t* restype, int32_t* hp, double* sv, double* boxl) {
+int pdmd_pair_codes(const pdmd_system_desc* d, int32_t n, const int32_t* a, const int32_t* b, int32_t* code) {
+    if (int e = validate_desc(d)) return e;
+    if (n < 0 || (n > 0 && (!a || !b || !code))) return fail("pdmd_pair_codes: bad arguments");
+    const int nc = d->nchains;
+    std::vector<int> clen(d->chain_len, d->chain_len + nc), spec(nc);
+    for (int c = 0; c < nc; ++c) spec[c] = d->species ? d->species[c] : c;
+    int nres = 0;
+    for (int c = 0; c < nc; ++c) nres += clen[c];
+    std::vector<int> rt(d->restype, d->restype + nres), hp(d->hp, d->hp + nres);
+    Topology t;
+    t.build(clen, spec, rt, hp);
+    for (int k = 0; k < n; ++k) {
+        if (a[k] < 1 || a[k] > t.N || b[k] < 1 || b[k] > t.N) return fail("pdmd_pair_codes: bead index out of range");
+        code[k] = a[k] == b[k] ? 1 : static_code(t, a[k] - 1, b[k] - 1);
+    }
+    return 0;
+}
+
+int pdmd_generate_seq(const char* param_dir, int32_t nchains, const int32_t* chain_len, const int32_t* restype_in,
+                      double density, double tstar, uint64_t seed, int32_t* restype, int32_t* hp, double* sv,
+                      double* boxl) {
     if (!param_dir || nchains <= 0 || !chain_len || !restype || !hp || !sv || !boxl)
         return fail("pdmd_generate: bad arguments");
     Params raw;
@@ -801,6 +829,11 @@ int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_l
     if (!load_params(raw, param_dir, err)) return fail(err);
     GenSpec g;
     g.chain_len.assign(chain_len, chain_len + nchains);
+    if (restype_in) {
+        int nres = 0;
+        for (int c = 0; c < nchains; ++c) nres += chain_len[c];
+        g.restype.assign(restype_in, restype_in + nres);
+    }
     g.density = density;
     g.tstar = tstar;
     g.seed = seed;
@@ -811,6 +844,11 @@ int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_l
     std::memcpy(sv, o.sv.data(), sizeof(double) * o.sv.size());
     *boxl = o.boxl;
     return 0;
+}
+
+int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_len, double density, double tstar,
+                  uint64_t seed, int32_t* restype, int32_t* hp, double* sv, double* boxl) {
+    return pdmd_generate_seq(param_dir, nchains, chain_len, nullptr, density, tstar, seed, restype, hp, sv, boxl);
 }
 
 }  // extern "C"

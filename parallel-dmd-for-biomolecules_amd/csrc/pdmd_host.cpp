@@ -459,7 +459,15 @@ bool generate(const Params& raw, const GenSpec& spec, GenOut& out, std::string& 
     }
     out.restype.assign(nres, 9);
     out.hp.assign(nres, 1);  // hp1.inp: every side chain hydrophobic-capable, as shipped
-    for (int q = 0; q < nres; ++q) out.restype[q] = 9 + (int)(rng.u() * 20.0) % 20;
+    if (!spec.restype.empty()) {
+        if ((int)spec.restype.size() != nres) { err = "given sequence length differs from the chain lengths"; return false; }
+        for (int q = 0; q < nres; ++q) {
+            if (spec.restype[q] < 9 || spec.restype[q] > 28) { err = "side-chain identities must lie in 9..28"; return false; }
+            out.restype[q] = spec.restype[q];
+        }
+    } else {
+        for (int q = 0; q < nres; ++q) out.restype[q] = 9 + (int)(rng.u() * 20.0) % 20;
+    }
     const double sig1 = raw.sigma[1];
     out.boxl = std::cbrt((double)N * sig1 * sig1 * sig1 / spec.density);
     const double B = out.boxl;

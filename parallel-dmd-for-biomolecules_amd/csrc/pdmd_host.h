@@ -65,6 +65,7 @@ struct GenSpec {
     double density = 0.01;
     double tstar = 0.2;
     uint64_t seed = 1;
+    std::vector<int> restype;  // optional given sequence (9..28 per residue); empty = drawn from the seed
 };
 struct GenOut {
     double boxl = 0;

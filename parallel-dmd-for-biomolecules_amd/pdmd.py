@@ -92,7 +92,7 @@ EXPORTS = [
     "pdmd_version", "pdmd_last_error", "pdmd_ctx_create", "pdmd_ctx_destroy", "pdmd_system_add",
     "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
     "pdmd_download", "pdmd_stats", "pdmd_log_enable", "pdmd_log_read", "pdmd_predict_pairs", "pdmd_observables",
-    "pdmd_generate", "pdmd_last_launch_ms",
+    "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
 ]
 
 _lib = None
@@ -126,6 +126,9 @@ def lib():
         L.pdmd_observables.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
         L.pdmd_generate.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
+        L.pdmd_pair_codes.argtypes = [ctypes.POINTER(SystemDesc), ctypes.c_int32, _i32p, _i32p, _i32p]
+        L.pdmd_generate_seq.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, _i32p, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -143,8 +146,11 @@ def _p(a, ctype):
     return a.ctypes.data_as(ctypes.POINTER(ctype)) if a is not None else None
 
 
-def generate(chain_len, density=0.01, tstar=0.20, seed=1, param_dir=PARAM_DIR):
-    """Seeded synthetic PRIME20 system (sequences, coils, MB velocities)."""
+def generate(chain_len, density=0.01, tstar=0.20, seed=1, param_dir=PARAM_DIR, sequence=None):
+    """Seeded synthetic PRIME20 system (sequences, coils, MB velocities).
+
+    sequence: optional side-chain identities (9..28) per residue, all chains
+    concatenated; None draws them from the seed."""
     cl = np.ascontiguousarray(chain_len, dtype=np.int32)
     nres = int(cl.sum())
     N = 4 * nres
@@ -152,11 +158,28 @@ def generate(chain_len, density=0.01, tstar=0.20, seed=1, param_dir=PARAM_DIR):
     hp = np.zeros(nres, np.int32)
     sv = np.zeros(6 * N, np.float64)
     boxl = ctypes.c_double(0.0)
-    _check(lib().pdmd_generate(param_dir.encode(), len(cl), _p(cl, ctypes.c_int32), density, tstar, seed,
-                               _p(restype, ctypes.c_int32), _p(hp, ctypes.c_int32), _p(sv, ctypes.c_double),
-                               ctypes.byref(boxl)))
+    seq = None
+    if sequence is not None:
+        seq = np.ascontiguousarray(sequence, dtype=np.int32)
+        if seq.size != nres:
+            raise ValueError("sequence length differs from sum(chain_len)")
+    _check(lib().pdmd_generate_seq(param_dir.encode(), len(cl), _p(cl, ctypes.c_int32),
+                                   None if seq is None else _p(seq, ctypes.c_int32), density, tstar, seed,
+                                   _p(restype, ctypes.c_int32), _p(hp, ctypes.c_int32), _p(sv, ctypes.c_double),
+                                   ctypes.byref(boxl)))
     return {"chain_len": cl, "restype": restype, "hp": hp, "sv": sv, "boxl": boxl.value, "tstar": tstar,
             "density": density}
+
+
+def pair_codes(system, a, b, species=None):
+    """Static pair codes ev_code(a, b) (1-based beads) of a system; host only."""
+    d, keep = Engine.make_desc(system, 0, species=species)
+    a = np.ascontiguousarray(a, np.int32)
+    b = np.ascontiguousarray(b, np.int32)
+    out = np.zeros(a.size, np.int32)
+    _check(lib().pdmd_pair_codes(ctypes.byref(d), a.size, _p(a, ctypes.c_int32), _p(b, ctypes.c_int32),
+                                 _p(out, ctypes.c_int32)))
+    return out
 
 
 class Engine:

@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference-built checker.
+
+Runs oracle/_ref/ref_harness (the reference's own Fortran subroutines,
+compiled by oracle/ref/Makefile from /root/reference) on a seeded two-species
+system, in a working directory of reference-format input files:
+
+  parameters/{protein,beadwell_ha55a,rcarnrco,sqz6to10,mass}.data and
+  parametersep/ep19p_ha55a_weakhp.data  -> the reference's own tables
+  parameters/identity.inp, hp1.inp, hp2.inp, firstside1.data,
+  firstside2.data, rn.data, rc.data, bdln.inp, helix.inp  -> written here
+  (formats read by inputinfo.f:28-66,105-262)
+
+and stores inputs and outputs as .npz (data only).  Run in this container
+after `make -C oracle/ref`; the GPU box never runs it.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "parallel-dmd-for-biomolecules_amd"))
+REF = "/root/reference/parallel-dmd-PRIME20"
+OUT = os.path.join(ROOT, "oracle", "_ref")
+
+# species 1: 2 chains x 8 residues, species 2: 2 chains x 6 residues
+# (the sizes oracle/ref/Makefile compiles the checker for)
+L1, L2, NCH1, NCH2 = 8, 6, 2, 2
+LOG_KEEP = 20000
+
+
+def write_inputs(work, sysd, seq1, seq2):
+    os.makedirs(os.path.join(work, "parameters"), exist_ok=True)
+    os.makedirs(os.path.join(work, "parametersep"), exist_ok=True)
+    os.makedirs(os.path.join(work, "results"), exist_ok=True)
+    for f in ("protein.data", "beadwell_ha55a.data", "rcarnrco.data", "sqz6to10.data", "mass.data"):
+        dst = os.path.join(work, "parameters", f)
+        if not os.path.lexists(dst):
+            os.symlink(os.path.join(REF, "parameters", f), dst)
+    dst = os.path.join(work, "parametersep", "ep19p_ha55a_weakhp.data")
+    if not os.path.lexists(dst):
+        os.symlink(os.path.join(REF, "parametersep", "ep19p_ha55a_weakhp.data"), dst)
+
+    def chain_ids(seq):
+        L = len(seq)
+        return [2] * L + [1] * L + [4] * L + list(seq)
+
+    ids = chain_ids(seq1) + chain_ids(seq2)
+    P = os.path.join(work, "parameters")
+    with open(os.path.join(P, "identity.inp"), "w") as f:
+        f.write(" ".join(str(x) for x in ids) + "\n")
+    with open(os.path.join(P, "hp1.inp"), "w") as f:
+        f.write(" ".join(["0"] * (3 * L1) + ["1"] * L1) + "\n")
+    with open(os.path.join(P, "hp2.inp"), "w") as f:
+        f.write(" ".join(["0"] * (3 * L2) + ["1"] * L2) + "\n")
+    with open(os.path.join(P, "firstside1.data"), "w") as f:
+        f.write(" ".join(["1"] * L1) + "\n")
+    with open(os.path.join(P, "firstside2.data"), "w") as f:
+        f.write(" ".join(["1"] * L2) + "\n")
+    # placeholders: inputinfo.f overwrites these from rcarnrco.data per residue
+    for name, val in (("rn.data", 1.45), ("rc.data", 1.52), ("bdln.inp", 2.4)):
+        with open(os.path.join(P, name), "w") as f:
+            f.write("\n".join(f"{val:.3f}" for _ in range(L1 + L2)) + "\n")
+    # -Drunh start: nn, boxl, x(N) y(N) z(N) in Angstrom, then vx vy vz
+    sv = sysd["sv"].reshape(-1, 6)
+    B = sysd["boxl"]
+    N = sv.shape[0]
+    with open(os.path.join(P, "helix.inp"), "w") as f:
+        f.write(f"{N}\n{B!r}\n")
+        for c in range(3):
+            f.write("\n".join(repr(float(x * B)) for x in sv[:, c]) + "\n")
+        for c in range(3, 6):
+            f.write("\n".join(repr(float(x)) for x in sv[:, c]) + "\n")
+    return np.array(ids, np.int32)
+
+
+def read_outputs(work, N, L1, L2):
+    o = {}
+    f = open(os.path.join(work, "setup.bin"), "rb").read()
+    d = np.frombuffer(f, np.uint8)
+    off = 0
+
+    def take(dtype, n):
+        nonlocal off
+        a = np.frombuffer(d[off:off + n * np.dtype(dtype).itemsize].tobytes(), dtype)
+        off += n * np.dtype(dtype).itemsize
+        return a
+
+    for k in ("sigma_sq", "welldia_sq", "ep_sqrt", "shlddia_sq"):
+        o[k] = take(np.float64, 28 * 28).reshape(28, 28, order="F")
+    o["ev_param"] = take(np.float64, 3 * 50).reshape(3, 50, order="F")
+    o["sqz610"] = take(np.float64, 5 * 28).reshape(5, 28, order="F")
+    o["bm"] = take(np.float64, N)
+    for k in ("bdln", "bl_rn", "bl_rc", "del_bdln", "del_blrn", "del_blrc"):
+        o[k] = take(np.float64, L1 + L2)
+    o["rlsq"] = take(np.float64, 50)
+    sc = take(np.float64, 8)
+    for k, v in zip(("sig_max_all", "hdelr", "width", "num_cell", "interval", "interval_max", "sortsize",
+                     "setemp"), sc):
+        o[k] = v
+    o["sv0"] = take(np.float64, 6 * N)
+    o["identity"] = take(np.int32, N)
+    o["chnnum"] = take(np.int32, N)
+    o["hp_bead"] = take(np.int32, 4 * (L1 + L2))
+    assert off == d.size, (off, d.size)
+    o["ev_code"] = np.fromfile(os.path.join(work, "evcode.bin"), np.int32).reshape(N, N, order="F")
+    nb = np.fromfile(os.path.join(work, "nb.bin"), np.int32)
+    o["na_npt"], o["nnabdn"] = nb[:N].copy(), nb[N:2 * N].copy()
+    o["nb_flat"] = nb[2 * N:2 * N + o["na_npt"].sum()].copy()
+    o["dn_flat"] = nb[2 * N + o["na_npt"].sum():].copy()
+    assert o["dn_flat"].size == o["nnabdn"].sum()
+    c = np.fromfile(os.path.join(work, "cal0.bin"), np.uint8)
+    o["tim0"] = np.frombuffer(c[:8 * (N + 3)].tobytes(), np.float64)
+    o["nptnr0"] = np.frombuffer(c[8 * (N + 3):12 * (N + 3)].tobytes(), np.int32)
+    o["coltype0"] = np.frombuffer(c[12 * (N + 3):16 * (N + 3)].tobytes(), np.int32)
+    p = np.fromfile(os.path.join(work, "pred.bin"), np.dtype([("tij", "<f8"), ("type", "<i4")]))
+    o["pred_tij"], o["pred_type"] = p["tij"].copy(), p["type"].copy()
+    ln = np.fromfile(os.path.join(work, "loopn.bin"), np.int32)
+    o["loop_n"], o["loop_pairs"], o["loop_updates"], o["loop_forced"] = (int(x) for x in ln[:4])
+    lp = np.fromfile(os.path.join(work, "loop.bin"), np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"),
+                                                               ("t", "<f8")]))
+    # the first LOG_KEEP entries are kept event by event; the final state
+    # pins the whole run
+    for k in ("i", "j", "type", "t"):
+        o["loop_" + k] = lp[k][:LOG_KEEP].copy()
+    fin = np.fromfile(os.path.join(work, "final.bin"), np.uint8)
+    q = 0
+    o["sv1"] = np.frombuffer(fin[q:q + 48 * N].tobytes(), np.float64); q += 48 * N
+    o["tim1"] = np.frombuffer(fin[q:q + 8 * (N + 3)].tobytes(), np.float64); q += 8 * (N + 3)
+    o["nptnr1"] = np.frombuffer(fin[q:q + 4 * (N + 3)].tobytes(), np.int32); q += 4 * (N + 3)
+    o["coltype1"] = np.frombuffer(fin[q:q + 4 * (N + 3)].tobytes(), np.int32); q += 4 * (N + 3)
+    o["tfalse1"] = np.frombuffer(fin[q:q + 8].tobytes(), np.float64)[0]
+    ob = np.fromfile(os.path.join(work, "obs.bin"), np.uint8).tobytes()
+    rec = np.dtype([("f", "<f8", 7), ("i", "<i4", 3)])
+    obs = np.frombuffer(ob, rec)
+    o["obs_f"], o["obs_i"] = obs["f"].copy(), obs["i"].copy()
+    return o
+
+
+def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000):
+    import pdmd
+    rng = np.random.default_rng(seed)
+    seq1 = [int(x) for x in rng.integers(9, 29, L1)]
+    seq2 = [int(x) for x in rng.integers(9, 29, L2)]
+    seq = seq1 * NCH1 + seq2 * NCH2
+    sysd = pdmd.generate([L1] * NCH1 + [L2] * NCH2, density=density, tstar=tstar, seed=seed, sequence=seq)
+    work = os.path.join(OUT, "work_" + name)
+    if os.path.isdir(work):
+        shutil.rmtree(work)
+    os.makedirs(work)
+    write_inputs(work, sysd, seq1, seq2)
+    r = subprocess.run([exe], input=f"{tstar}\n{nsteps}\n", cwd=work, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"{exe} failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    print(name, r.stdout.strip().splitlines()[-1])
+    N = 4 * (NCH1 * L1 + NCH2 * L2)
+    o = read_outputs(work, N, L1, L2)
+    o.update(chain_len=np.array([L1] * NCH1 + [L2] * NCH2, np.int32), restype=np.array(seq, np.int32),
+             hp=sysd["hp"], boxl=sysd["boxl"], tstar=tstar, density=density, species=np.array([0] * NCH1 + [1] * NCH2,
+                                                                                               np.int32))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **o)
+    shutil.rmtree(work)
+    return o
+
+
+def main():
+    exe = os.path.join(OUT, "ref_harness")
+    exe_n = os.path.join(OUT, "ref_harness_nohbs")
+    if not (os.path.exists(exe) and os.path.exists(exe_n)):
+        raise SystemExit("build the checker first: make -C oracle/ref")
+    run_case("ref_2species", exe, seed=4242)
+    run_case("ref_2species_nohbs", exe_n, seed=4243)
+
+
+if __name__ == "__main__":
+    main()

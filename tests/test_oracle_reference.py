@@ -1,0 +1,142 @@
+"""CPU oracle against vectors written by the reference's own subroutines.
+
+The fixtures in tests/golden/ come from oracle/_ref/ref_harness (the
+reference's unmodified Fortran compiled by oracle/ref/Makefile, driven by
+tests/golden/make_golden.py) on a two-species system (2 chains x 8 residues
+and 2 chains x 6 residues, rho* = 0.01, T* = 0.20): the scaled parameter
+tables (inputinfo.f + scale_down.f), ev_code (make_code.f), the neighbour
+lists (nbor_setup.f, cell_link.f, nbor.f), the calendar after events.f,
+every pair prediction (core/bond/sqwel/sqshlder/nc_sqwel.f), the energy and
+shape observables (energy.f, radgyr.f, end2end.f) and a serial run of
+eventdyn.f + partial_events.f + the calendar + the sync pseudo-event in the
+reference's order.  Everything is compared bit for bit, observables to
+1e-12 relative (their sums run in a different order).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = [("ref_2species", False), ("ref_2species_nohbs", True)]
+
+
+def _load(name):
+    return dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
+
+
+def _oracle(fx, param_dir, no_hbs):
+    sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"],
+                boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
+    return Oracle(sysd, seed=1, param_dir=param_dir, canon=False, species=fx["species"], no_hbs=no_hbs)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.int64)
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request, param_dir):
+    name, no_hbs = request.param
+    fx = _load(name)
+    return fx, _oracle(fx, param_dir, no_hbs), no_hbs
+
+
+def test_tables(case):
+    fx, orc, _ = case
+    for w, key in enumerate(("sigma_sq", "welldia_sq", "ep_sqrt", "shlddia_sq")):
+        got = np.array([[orc.table(w, k, kk) for kk in range(1, 29)] for k in range(1, 29)])
+        assert np.array_equal(_bits(got), _bits(fx[key])), key
+    got = np.array([[orc.table(6, r, c) for c in range(1, 51)] for r in range(1, 4)])
+    assert np.array_equal(_bits(got), _bits(fx["ev_param"]))
+    got = np.array([[orc.table(7, r, c) for c in range(9, 29)] for r in range(1, 6)])
+    assert np.array_equal(_bits(got), _bits(fx["sqz610"][:, 8:]))
+    got = np.array([orc.table(8, k) for k in range(1, 51)])
+    assert np.array_equal(_bits(got), _bits(fx["rlsq"]))
+    st = orc.stats()
+    assert st.sig_max_all == fx["sig_max_all"] and st.hdelr == fx["hdelr"]
+    assert st.interval == fx["interval"] and st.interval_max == fx["interval_max"]
+    assert st.sortsize == fx["sortsize"] and st.num_cell == int(fx["num_cell"])
+    ident = orc.state()["identity"]
+    assert np.array_equal(ident, fx["identity"])
+    bm = np.array([orc.table(5, int(t)) for t in ident])
+    assert np.array_equal(_bits(bm), _bits(fx["bm"]))
+
+
+def test_ev_code(case):
+    fx, orc, no_hbs = case
+    N = orc.N
+    got = np.array([[orc.ev_code(a, b) for b in range(1, N + 1)] for a in range(1, N + 1)], np.int32)
+    ref = fx["ev_code"]
+    bad = np.argwhere(got != ref)
+    assert bad.size == 0, f"{len(bad)} codes differ, first (a,b)={bad[0] + 1}: oracle {got[tuple(bad[0])]} ref {ref[tuple(bad[0])]}"
+    assert (15 in ref) != no_hbs
+
+
+def test_neighbour_lists_in_reference_order(case):
+    fx, orc, _ = case
+    N = orc.N
+    up = np.split(fx["nb_flat"], np.cumsum(fx["na_npt"])[:-1])
+    dn = np.split(fx["dn_flat"], np.cumsum(fx["nnabdn"])[:-1])
+    for i in range(1, N + 1):
+        assert np.array_equal(orc.nb_up(i), up[i - 1]), f"up list of bead {i}"
+        assert np.array_equal(orc.nb_dn(i), dn[i - 1]), f"down list of bead {i}"
+
+
+def test_calendar_after_events(case):
+    fx, orc, _ = case
+    st = orc.state()
+    assert np.array_equal(_bits(st["tim"]), _bits(fx["tim0"]))
+    assert np.array_equal(st["nptnr"], fx["nptnr0"])
+    assert np.array_equal(st["coltype"], fx["coltype0"])
+
+
+def test_every_pair_prediction(case):
+    fx, orc, _ = case
+    N = orc.N
+    k = 0
+    for i in range(1, N):
+        for j in range(i + 1, N + 1):
+            t, ty = orc.predict(i, j)
+            rt = fx["pred_tij"][k]
+            assert np.float64(t).view(np.int64) == np.float64(rt).view(np.int64), (i, j, t, rt)
+            if rt < 1e9:
+                assert ty == fx["pred_type"][k], (i, j)
+            k += 1
+    assert k == fx["pred_tij"].size
+
+
+def _obs_close(o, f, iv):
+    got = np.array([o.ered, o.tred, o.sumvel, o.ehh_ii, o.ehh_ij, o.rg_avg, o.e2e_avg])
+    np.testing.assert_allclose(got, f, rtol=1e-12, atol=1e-300)
+    assert (o.hb_alpha, o.hb_ii, o.hb_ij) == tuple(int(x) for x in iv)
+
+
+def test_observables_initial(case):
+    fx, orc, _ = case
+    _obs_close(orc.observables(), fx["obs_f"][0], fx["obs_i"][0])
+
+
+def test_serial_loop(param_dir):
+    # fresh oracles: the module fixture's ones have been queried only
+    for name, no_hbs in CASES:
+        fx = _load(name)
+        orc = _oracle(fx, param_dir, no_hbs)
+        n = int(fx["loop_n"])
+        keep = fx["loop_i"].size
+        log = orc.run(n, log_cap=keep)
+        for k in ("i", "j", "type"):
+            bad = np.nonzero(log[k] != fx["loop_" + k])[0]
+            assert bad.size == 0, f"{name}: {k} differs first at entry {bad[0] + 1}"
+        assert np.array_equal(_bits(log["t"]), _bits(fx["loop_t"])), name
+        st = orc.state()
+        assert np.array_equal(_bits(st["sv"]), _bits(fx["sv1"])), name
+        assert np.array_equal(_bits(st["tim"]), _bits(fx["tim1"])), name
+        assert np.array_equal(st["nptnr"], fx["nptnr1"]), name
+        assert np.array_equal(st["coltype"], fx["coltype1"]), name
+        s = orc.stats()
+        assert s.nupdates == fx["loop_updates"] and s.nforced == fx["loop_forced"], name
+        assert s.tfalse == fx["tfalse1"], name
+        _obs_close(orc.observables(), fx["obs_f"][1], fx["obs_i"][1])
