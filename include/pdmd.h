@@ -54,6 +54,8 @@ typedef struct {
     int32_t status, max_list;
     int32_t nbeads, pad;
     double avg_up, avg_dn; /* mean up/down neighbour-list lengths (K_up, K_dn) */
+    int64_t phase_cycles[32]; /* device cycles per engine phase; zero unless the
+                                 library is built with -DPDMD_PHASE_TIMERS */
 } pdmd_stats_t;
 
 typedef struct {

@@ -58,7 +58,8 @@ struct HostSys {
     DevBuf<int> nb_up, nup, nb_dn, ndn, cfine, ckey, cstart, crun, csorted;
     DevBuf<int4> extra, ovr;
     DevBuf<Leaf> leaf;
-    DevBuf<double4> old_r;
+    DevBuf<double4> old_r, spos;
+    DevBuf<int4> sinfo;
     DevBuf<SysState> st;
     DevBuf<LogRec> log;
     SysDev dev{};
@@ -215,7 +216,7 @@ int64_t pdmd_system_bytes(const pdmd_system_desc* d) {
     int64_t b = N * (int64_t)sizeof(Rec) + N * 16 /*bw*/ + 2 * N * 16 /*extra, ovr*/ + N * 32 /*old_r*/ +
                 2 * N * cap * 4 + 2 * N * 4 + (N + 3) * 20 + nres * 48 + (int64_t)sizeof(DevTables);
     // coarse grid upper bound: box/rl cells per axis at the densest tables
-    b += 4 * N * 4 + 2 * 4 * 1024 * 1024;
+    b += 4 * N * 4 + N * 48 + 2 * 4 * 1024 * 1024;
     return b;
 }
 
@@ -384,6 +385,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     HIPCHK(H.cfine.alloc(N));
     HIPCHK(H.ckey.alloc(N));
     HIPCHK(H.csorted.alloc(N));
+    HIPCHK(H.spos.alloc(N));
+    HIPCHK(H.sinfo.alloc(N));
     HIPCHK(H.cstart.alloc((size_t)Mq * Mq * Mq + 1));
     HIPCHK(H.crun.alloc((size_t)Mq * Mq * Mq));
     {
@@ -421,6 +424,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.cstart = H.cstart.p;
     D.crun = H.crun.p;
     D.csorted = H.csorted.p;
+    D.spos = H.spos.p;
+    D.sinfo = H.sinfo.p;
     D.st = H.st.p;
     D.log = nullptr;
     H.alive = true;
@@ -449,7 +454,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     if (ctx->d_ids.n < 1) HIPCHK(ctx->d_ids.alloc(4096));
     if (ctx->d_inc.n < 1) HIPCHK(ctx->d_inc.alloc(4096));
     HIPCHK(hipMemcpy(ctx->d_ids.p, &id, sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, 4 * H.cap + 16, ctx->stream));
+    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, engine_maxw(H.cap), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     SysState chk;
     HIPCHK(hipMemcpy(&chk, H.st.p, sizeof chk, hipMemcpyDeviceToHost));
@@ -501,7 +506,7 @@ int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const in
     HIPCHK(hipMemcpyAsync(ctx->d_ids.p, dids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_inc.p, inc.data(), sizeof(long long) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, 4 * max_cap + 16, ctx->lds_limit, ctx->stream));
+    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     // the copies above read pageable memory synchronously before returning
@@ -612,6 +617,7 @@ int pdmd_stats(pdmd_ctx* ctx, int32_t id, pdmd_stats_t* out) {
     }
     out->avg_up = (double)su / H->N;
     out->avg_dn = (double)sd / H->N;
+    for (int k = 0; k < 32; ++k) out->phase_cycles[k] = st.phase[k];
     return 0;
 }
 

@@ -72,6 +72,7 @@ struct SysState {
     int nbin, status, log_count, pad1;
     long long target;
     long long max_nb_seen;
+    long long phase[32];  // device cycles per engine phase (PDMD_PHASE_TIMERS builds only)
 };
 
 struct LogRec {
@@ -101,6 +102,8 @@ struct SysDev {
     int* cstart;   // [ncoarse+1]
     int* crun;     // [ncoarse]
     int* csorted;  // [N]
+    double4* spos; // [N] positions in coarse-cell order (list rebuild)
+    int4* sinfo;   // [N] bead, topo word, fine cell, 0 in coarse-cell order
     SysState* st;
     LogRec* log;
 };

@@ -9,6 +9,8 @@ namespace pdmd {
 // LDS bytes of one event-loop wavefront for a calendar of nblk 16-entry
 // blocks and a down-neighbour work list of maxd entries.
 size_t engine_lds_bytes(int nblk, int maxw, int leaf_cap);
+// work-list capacity of the event kernels for a list capacity
+int engine_maxw(int cap);
 // Opt the event-loop kernels in to lds_limit bytes of LDS per workgroup.
 hipError_t engine_prepare(size_t lds_limit);
 // Lists, pseudo-event seeds and events() for n systems (one wavefront each).

@@ -32,6 +32,46 @@ constexpr int kMaxBlist = 256;
 constexpr double kLt = 1e-10;
 constexpr double kSm = 5e-12;
 
+// Phase timers (device clock cycles summed into SysState::phase[k]) when
+// built with -DPDMD_PHASE_TIMERS; no code otherwise.  Phases: 0 selection,
+// 1 pair event before rescheduling, 2 reschedule loads and work list,
+// 3 reschedule predictions, 4 down-set updates, 5 leaf writes, 6 block
+// rescans, 7 sync event without rebuild, 8 list rebuild, 9 full events(),
+// 10 ghost event before rescheduling, 11 reschedule rows and work list,
+// 12 reschedule down-set scan, 13 rebuild count, 14 sync count; list
+// rebuild steps: 16 cells and histogram, 17 scan, 18 scatter, 19 sorted
+// copies, 20 neighbour search.
+enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
+       kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
+       kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20 };
+#ifdef PDMD_PHASE_TIMERS
+#define PH_BEGIN(v) long long v = clock64()
+#define PH_END(S, k, v)                                                                                  \
+    do {                                                                                                 \
+        if (threadIdx.x == 0)                                                                            \
+            atomicAdd((unsigned long long*)&(S).st->phase[k], (unsigned long long)(clock64() - (v)));   \
+        v = clock64();                                                                                   \
+    } while (0)
+#define PH_COUNT(S, k)                                                                      \
+    do {                                                                                    \
+        if (threadIdx.x == 0) atomicAdd((unsigned long long*)&(S).st->phase[k], 1ull);     \
+    } while (0)
+#define PH_RESET(v) v = clock64()
+#else
+#define PH_BEGIN(v) \
+    do {            \
+    } while (0)
+#define PH_END(S, k, v) \
+    do {                \
+    } while (0)
+#define PH_COUNT(S, k) \
+    do {               \
+    } while (0)
+#define PH_RESET(v) \
+    do {            \
+    } while (0)
+#endif
+
 // ------------------------------------------------------------------------
 // project RNG (counter-based splitmix64) and a logarithm from basic ops
 // ------------------------------------------------------------------------
@@ -344,20 +384,56 @@ __device__ inline Rec shfl_rec(const Rec& r, int src) {
     o.dyn = __shfl(r.dyn, src, kW);
     return o;
 }
-__device__ inline int wave_or(int v) {
-    for (int s = 1; s < kW; s <<= 1) v |= __shfl_xor(v, s, kW);
-    return v;
+// Hand-off through LDS between lanes of the workgroup, which is one
+// wavefront: LDS operations of a wavefront complete in program order, so
+// only the compiler's ordering is needed (no wait for outstanding global
+// loads, unlike __syncthreads).  Global-memory hand-offs keep
+// __threadfence_block + __syncthreads.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// DPP lane moves: CTRL row_shr:n = 0x110+n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143; lanes without a valid source keep their own value
+template <int CTRL, int RM>
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ inline double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL, RM>((int)b), hi = dpp_i<CTRL, RM>((int)(b >> 32));
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+__device__ inline double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+// first non-zero status of the wave (0 if none); wave-uniform
+__device__ inline int wave_status(int v) {
+    const unsigned long long m = __ballot(v != 0);
+    return m ? __builtin_amdgcn_readlane(v, __ffsll((long long)m) - 1) : 0;
 }
 __device__ inline double wave_max(double v) {
-    for (int s = 1; s < kW; s <<= 1) {
-        const double o = __shfl_xor(v, s, kW);
-        v = o > v ? o : v;
-    }
-    return v;
+    double o;
+    o = dpp_d<0x111, 0xf>(v); v = o > v ? o : v;
+    o = dpp_d<0x112, 0xf>(v); v = o > v ? o : v;
+    o = dpp_d<0x114, 0xf>(v); v = o > v ? o : v;
+    o = dpp_d<0x118, 0xf>(v); v = o > v ? o : v;
+    o = dpp_d<0x142, 0xa>(v); v = o > v ? o : v;
+    o = dpp_d<0x143, 0xc>(v); v = o > v ? o : v;
+    return readlane_d(v, 63);
 }
 __device__ inline int wave_imax(int v) {
-    for (int s = 1; s < kW; s <<= 1) v = max(v, __shfl_xor(v, s, kW));
-    return v;
+    v = max(v, dpp_i<0x111, 0xf>(v));
+    v = max(v, dpp_i<0x112, 0xf>(v));
+    v = max(v, dpp_i<0x114, 0xf>(v));
+    v = max(v, dpp_i<0x118, 0xf>(v));
+    v = max(v, dpp_i<0x142, 0xa>(v));
+    v = max(v, dpp_i<0x143, 0xc>(v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 // min of predictions (t, position) carrying partner and type
 __device__ inline void wave_min_pred(double& t, int& p, int& m, int& ty) {
@@ -375,35 +451,65 @@ __device__ inline void wave_min_pred(double& t, int& p, int& m, int& ty) {
     }
 }
 
-// calendar tree entries
-struct BEnt {
+// Calendar key of one entry: order (bin, time, index), payload partner and
+// type.  bc = bin << 8 | (coltype + 2); ip = index << 16 | (partner + 2).
+struct Key {
     double t;
-    int bin, idx, p, ct;
+    int bc;
+    unsigned ip;
 };
-__device__ inline bool key_less(int b1, double t1, int i1, int b2, double t2, int i2) {
-    if (b1 != b2) return b1 < b2;
-    if (t1 != t2) return t1 < t2;
-    return i1 < i2;
+constexpr int kEmptyBc = INT_MAX;
+__device__ inline bool key_less(const Key& a, const Key& b) {
+    const int ab = a.bc >> 8, bb = b.bc >> 8;
+    if (ab != bb) return ab < bb;
+    if (a.t != b.t) return a.t < b.t;
+    return a.ip < b.ip;
 }
-__device__ inline void red_key(BEnt& k, int width) {
-    for (int m = 1; m < width; m <<= 1) {
-        BEnt o;
-        o.t = __shfl_xor(k.t, m, kW);
-        o.bin = __shfl_xor(k.bin, m, kW);
-        o.idx = __shfl_xor(k.idx, m, kW);
-        o.p = __shfl_xor(k.p, m, kW);
-        o.ct = __shfl_xor(k.ct, m, kW);
-        if (key_less(o.bin, o.t, o.idx, k.bin, k.t, k.idx)) k = o;
-    }
-}
-__device__ inline BEnt empty_ent() {
-    BEnt e;
+__device__ inline Key empty_key() {
+    Key e;
     e.t = 0.0;
-    e.bin = INT_MAX;
-    e.idx = INT_MAX;
-    e.p = -1;
-    e.ct = 0;
+    e.bc = kEmptyBc;
+    e.ip = 0xFFFFFFFFu;
     return e;
+}
+__device__ inline int key_bin(const Key& k) { return k.bc >> 8; }
+__device__ inline int key_ct(const Key& k) { return (k.bc & 0xff) - 2; }
+__device__ inline int key_idx(const Key& k) { return (int)(k.ip >> 16); }
+__device__ inline int key_p(const Key& k) { return (int)(k.ip & 0xFFFFu) - 2; }
+__device__ inline Key leaf_key(const Leaf& lf, int idx) {
+    const int bin = leaf_bin(lf.meta);
+    if (bin < 0) return empty_key();
+    Key k;
+    k.t = lf.tim;
+    k.bc = (bin << 8) | (leaf_ct(lf.meta) + 2);
+    k.ip = ((unsigned)idx << 16) | (unsigned)(lf.nptnr + 2);
+    return k;
+}
+template <int CTRL, int RM>
+__device__ inline void key_step(Key& k) {
+    Key o;
+    o.t = dpp_d<CTRL, RM>(k.t);
+    o.bc = dpp_i<CTRL, RM>(k.bc);
+    o.ip = (unsigned)dpp_i<CTRL, RM>((int)k.ip);
+    if (key_less(o, k)) k = o;
+}
+// minimum over each row of 16 lanes, in the row's last lane
+__device__ inline void key_min_rows(Key& k) {
+    key_step<0x111, 0xf>(k);
+    key_step<0x112, 0xf>(k);
+    key_step<0x114, 0xf>(k);
+    key_step<0x118, 0xf>(k);
+}
+// minimum over the wave, returned to every lane
+__device__ inline Key key_min_wave(Key k) {
+    key_min_rows(k);
+    key_step<0x142, 0xa>(k);
+    key_step<0x143, 0xc>(k);
+    Key r;
+    r.t = readlane_d(k.t, 63);
+    r.bc = __builtin_amdgcn_readlane(k.bc, 63);
+    r.ip = (unsigned)__builtin_amdgcn_readlane((int)k.ip, 63);
+    return r;
 }
 
 // per-system register state
@@ -434,83 +540,109 @@ struct Pair {  // the event's colliding pair (j < 0 for a ghost event)
     int4 xi, xj;
 };
 
+constexpr int kMaxSpec = 8;  // full recomputes batched with the pair's own
+// per-owner minimum of a prediction batch: LDS atomics on (time, position)
+struct MinSlot {
+    unsigned long long tkey;
+    unsigned packed;  // position << 21 | (type + 1) << 16 | partner
+    int pad;
+};
+// a down-set bead whose full recompute is batched with the pair's
+struct SpecInfo {
+    Rec r;
+    double2 bw;
+    int4 x;
+    int l, nu, nx, off;
+};
+// time as an unsigned key with the order of the doubles (-0 as +0)
+__device__ inline unsigned long long t_order(double t) {
+    if (t == 0.0) t = 0.0;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(t);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// The workgroup's dynamic LDS.  Views into it are kept as offsets and
+// turned into pointers from this array where they are used, so the compiler
+// keeps LDS accesses as ds_* operations (a pointer stored in a struct and
+// passed on would become a flat pointer).
+extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+
 // LDS views
 struct Lds {
-    DevTables* tab;
-    BEnt* blk;
-    BEnt* sup;
-    int* wm;     // work list: partner / down bead
-    int* wx;     // work list: owner / op tag
-    int* blist;  // calendar blocks to rescan
-    Pair* pair;  // the current event's pair (wave-uniform, kept out of registers)
-    Sc* sc;      // the system's scalars (wave-uniform, kept out of registers)
-    Leaf* leaf;  // calendar leaves, in LDS or HBM
+    int o_blk, o_sup, o_wm, o_wx, o_blist, o_pair, o_sc, o_leaf, o_wd, o_wt, o_wy, o_slot, o_spec;
+    Leaf* gleaf;  // calendar leaves in HBM when they do not fit in LDS
     int nblk, nsup, maxw;
     bool leaf_lds;
+    __device__ Key* blk() const { return reinterpret_cast<Key*>(smem + o_blk); }
+    __device__ Key* sup() const { return reinterpret_cast<Key*>(smem + o_sup); }
+    __device__ int* wm() const { return reinterpret_cast<int*>(smem + o_wm); }  // work list: partner / bead
+    __device__ int* wx() const { return reinterpret_cast<int*>(smem + o_wx); }  // work list: owner / op tag
+    __device__ int* wd() const { return reinterpret_cast<int*>(smem + o_wd); }
+    __device__ double* wt() const { return reinterpret_cast<double*>(smem + o_wt); }  // predicted times
+    __device__ int* wy() const { return reinterpret_cast<int*>(smem + o_wy); }  // (type + 1) << 16 | partner
+    __device__ int* blist() const { return reinterpret_cast<int*>(smem + o_blist); }  // blocks to rescan
+    __device__ MinSlot* slot() const { return reinterpret_cast<MinSlot*>(smem + o_slot); }
+    __device__ SpecInfo* spec() const { return reinterpret_cast<SpecInfo*>(smem + o_spec); }
+    __device__ Pair* pair() const { return reinterpret_cast<Pair*>(smem + o_pair); }
+    __device__ Sc* sc() const { return reinterpret_cast<Sc*>(smem + o_sc); }
+    __device__ Leaf* lleaf() const { return reinterpret_cast<Leaf*>(smem + o_leaf); }
 };
+// the scaled tables sit at the start of the LDS
+__device__ inline const DevTables& tabs() { return *reinterpret_cast<const DevTables*>(smem); }
+__device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : L.gleaf[k]; }
+__device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
+    if (L.leaf_lds)
+        L.lleaf()[k] = v;
+    else
+        L.gleaf[k] = v;
+}
 
+// block minima of the listed blocks (4 blocks per pass, one row of 16 lanes
+// each), then the superblock minima over the touched superblocks
 __device__ void rescan_block_list(const Lds& L, int N3, int nb) {
     const int lane = threadIdx.x;
     for (int base = 0; base < nb; base += 4) {
         const int q = base + (lane >> 4), k = lane & 15;
-        const int b = q < nb ? L.blist[q] : -1;
-        BEnt e = empty_ent();
+        const int b = q < nb ? L.blist()[q] : -1;
+        Key e = empty_key();
         const int idx = b * kBlk + k;
-        if (b >= 0 && idx < N3) {
-            const Leaf lf = L.leaf[idx];
-            const int bin = leaf_bin(lf.meta);
-            if (bin >= 0) {
-                e.t = lf.tim;
-                e.bin = bin;
-                e.idx = idx;
-                e.p = lf.nptnr;
-                e.ct = leaf_ct(lf.meta);
-            }
-        }
-        red_key(e, 16);
-        if (k == 0 && b >= 0) L.blk[b] = e;
+        if (b >= 0 && idx < N3) e = leaf_key(leaf_get(L, idx), idx);
+        key_min_rows(e);
+        if (k == 15 && b >= 0) L.blk()[b] = e;
     }
-    __syncthreads();
-    // superblocks of the touched blocks
-    for (int q = 0; q < nb; ++q) {
-        const int s = L.blist[q] / kSup;
-        bool seen = false;
-        for (int r = 0; r < q; ++r)
-            if (L.blist[r] / kSup == s) seen = true;
-        if (seen) continue;
+    wave_sync();
+    // the touched superblocks (at most 64), each reduced once
+    unsigned long long touched = 0;
+    for (int base = 0; base < nb; base += kW) {
+        const int q = base + lane;
+        const int sq = q < nb ? L.blist()[q] / kSup : 0;
+        const int nr = min(nb - base, kW);
+        for (int r = 0; r < nr; ++r) touched |= 1ull << __builtin_amdgcn_readlane(sq, r);
+    }
+    while (touched) {
+        const int s = __ffsll((long long)touched) - 1;
+        touched &= touched - 1;
         const int b = s * kSup + lane;
-        BEnt e = b < L.nblk ? L.blk[b] : empty_ent();
-        red_key(e, kW);
-        if (lane == 0) L.sup[s] = e;
+        const Key e = key_min_wave(b < L.nblk ? L.blk()[b] : empty_key());
+        if (lane == 0) L.sup()[s] = e;
     }
-    __syncthreads();
+    wave_sync();
 }
 __device__ void rescan_all(const Lds& L, int N3) {
     const int lane = threadIdx.x;
     for (int base = 0; base < L.nblk; base += 4) {
         const int b = base + (lane >> 4), k = lane & 15;
-        BEnt e = empty_ent();
+        Key e = empty_key();
         const int idx = b * kBlk + k;
-        if (b < L.nblk && idx < N3) {
-            const Leaf lf = L.leaf[idx];
-            const int bin = leaf_bin(lf.meta);
-            if (bin >= 0) {
-                e.t = lf.tim;
-                e.bin = bin;
-                e.idx = idx;
-                e.p = lf.nptnr;
-                e.ct = leaf_ct(lf.meta);
-            }
-        }
-        red_key(e, 16);
-        if (k == 0 && b < L.nblk) L.blk[b] = e;
+        if (b < L.nblk && idx < N3) e = leaf_key(leaf_get(L, idx), idx);
+        key_min_rows(e);
+        if (k == 15 && b < L.nblk) L.blk()[b] = e;
     }
     __syncthreads();
     for (int s = 0; s < L.nsup; ++s) {
         const int b = s * kSup + lane;
-        BEnt e = b < L.nblk ? L.blk[b] : empty_ent();
-        red_key(e, kW);
-        if (lane == 0) L.sup[s] = e;
+        const Key e = key_min_wave(b < L.nblk ? L.blk()[b] : empty_key());
+        if (lane == 0) L.sup()[s] = e;
     }
     __syncthreads();
 }
@@ -526,8 +658,10 @@ __device__ inline Rec load_rec(const SysDev& S, const Pair& P, int m) {
 
 // Cooperative full recompute of bead l (rl, bwl, xl wave-uniform): min over
 // l's up list and upper shoulder partners (partial_events.f:121-139).
-__device__ void full_one(const SysDev& S, const DevTables& T, const Pair& P, int l, const Rec& rl, double2 bwl,
-                         int4 xl, double tf, double T0, double& tim, int& np, int& ct, int& status) {
+__device__ void full_one(const SysDev& S, const Lds& L, int l, const Rec& rl, double2 bwl, int4 xl, double tf,
+                         double T0, double& tim, int& np, int& ct, int& status) {
+    const DevTables& T = tabs();
+    const Pair& P = *L.pair();
     const int lane = threadIdx.x, cap = S.cap;
     const int nu = S.nup[l];
     const int nx = (xl.x > l ? 1 : 0) + (xl.y > l ? 1 : 0) + (xl.z > l ? 1 : 0);
@@ -607,14 +741,20 @@ __device__ inline void single_update(const DevTables& T, const Rec& rl, int4 ol,
     }
 }
 
+// Generic form for any down-set size: full recomputes one at a time.
 // extra_blk: a further calendar block to rescan (the ghost entry), or -1
-__device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc& c, const Pair& P, int extra_blk,
-                           bool xpulse_del) {
+__device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds& L, Sc&, const Pair&,
+                                   int extra_blk, bool xpulse_del) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+    const Pair& P = *L.pair();
+
     const int lane = threadIdx.x, cap = S.cap, N3 = S.N + 3;
     const int i = P.i, j = P.j;
     const double tf = c.tfalse;
     const double T0 = c.interval_max + kLt - tf;
     int status = 0;
+    PH_BEGIN(tph);
     // ---- list rows of i and j, one entry per lane (cap <= 64)
     const int nui = S.nup[i], ndi = S.ndn[i];
     const int nuj = j >= 0 ? S.nup[j] : 0, ndj = j >= 0 ? S.ndn[j] : 0;
@@ -628,58 +768,59 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
     const int xj0 = P.xj.x, xj1 = P.xj.y, xj2 = P.xj.z;
     int w = 0;
     if (lane < nui) {
-        L.wm[lane] = eui;
-        L.wx[lane] = 0;
+        L.wm()[lane] = eui;
+        L.wx()[lane] = 0;
     }
     w = nui;
     const int kNoStat = 0xFF << 24;
-    if (xi0 > i) { if (lane == 0) { L.wm[w] = xi0 | kNoStat; L.wx[w] = 0; } ++w; }
-    if (xi1 > i) { if (lane == 0) { L.wm[w] = xi1 | kNoStat; L.wx[w] = 0; } ++w; }
-    if (xi2 > i) { if (lane == 0) { L.wm[w] = xi2 | kNoStat; L.wx[w] = 0; } ++w; }
+    if (xi0 > i) { if (lane == 0) { L.wm()[w] = xi0 | kNoStat; L.wx()[w] = 0; } ++w; }
+    if (xi1 > i) { if (lane == 0) { L.wm()[w] = xi1 | kNoStat; L.wx()[w] = 0; } ++w; }
+    if (xi2 > i) { if (lane == 0) { L.wm()[w] = xi2 | kNoStat; L.wx()[w] = 0; } ++w; }
     if (j >= 0) {
         if (lane < nuj) {
-            L.wm[w + lane] = euj;
-            L.wx[w + lane] = 1;
+            L.wm()[w + lane] = euj;
+            L.wx()[w + lane] = 1;
         }
         w += nuj;
-        if (xj0 > j) { if (lane == 0) { L.wm[w] = xj0 | kNoStat; L.wx[w] = 1; } ++w; }
-        if (xj1 > j) { if (lane == 0) { L.wm[w] = xj1 | kNoStat; L.wx[w] = 1; } ++w; }
-        if (xj2 > j) { if (lane == 0) { L.wm[w] = xj2 | kNoStat; L.wx[w] = 1; } ++w; }
+        if (xj0 > j) { if (lane == 0) { L.wm()[w] = xj0 | kNoStat; L.wx()[w] = 1; } ++w; }
+        if (xj1 > j) { if (lane == 0) { L.wm()[w] = xj1 | kNoStat; L.wx()[w] = 1; } ++w; }
+        if (xj2 > j) { if (lane == 0) { L.wm()[w] = xj2 | kNoStat; L.wx()[w] = 1; } ++w; }
     }
     const int ncand = w;
     if (lane < ndi) {
-        L.wm[w + lane] = edi & 0xFFFFFF;
-        L.wx[w + lane] = 2;
+        L.wm()[w + lane] = edi & 0xFFFFFF;
+        L.wx()[w + lane] = 2;
     }
     w += ndi;
-    if (xi0 >= 0 && xi0 < i) { if (lane == 0) { L.wm[w] = xi0; L.wx[w] = 2; } ++w; }
-    if (xi1 >= 0 && xi1 < i) { if (lane == 0) { L.wm[w] = xi1; L.wx[w] = 2; } ++w; }
-    if (xi2 >= 0 && xi2 < i) { if (lane == 0) { L.wm[w] = xi2; L.wx[w] = 2; } ++w; }
+    if (xi0 >= 0 && xi0 < i) { if (lane == 0) { L.wm()[w] = xi0; L.wx()[w] = 2; } ++w; }
+    if (xi1 >= 0 && xi1 < i) { if (lane == 0) { L.wm()[w] = xi1; L.wx()[w] = 2; } ++w; }
+    if (xi2 >= 0 && xi2 < i) { if (lane == 0) { L.wm()[w] = xi2; L.wx()[w] = 2; } ++w; }
     if (j >= 0) {
         const int l = edj & 0xFFFFFF;
         const bool keep = lane < ndj && l != i;
         const unsigned long long bal = __ballot(keep);
         const int pre = __popcll(bal & ((1ull << lane) - 1ull));
         if (keep) {
-            L.wm[w + pre] = l;
-            L.wx[w + pre] = 3;
+            L.wm()[w + pre] = l;
+            L.wx()[w + pre] = 3;
         }
         w += __popcll(bal);
-        if (xj0 >= 0 && xj0 < j) { if (lane == 0) { L.wm[w] = xj0; L.wx[w] = 3; } ++w; }
-        if (xj1 >= 0 && xj1 < j) { if (lane == 0) { L.wm[w] = xj1; L.wx[w] = 3; } ++w; }
-        if (xj2 >= 0 && xj2 < j) { if (lane == 0) { L.wm[w] = xj2; L.wx[w] = 3; } ++w; }
+        if (xj0 >= 0 && xj0 < j) { if (lane == 0) { L.wm()[w] = xj0; L.wx()[w] = 3; } ++w; }
+        if (xj1 >= 0 && xj1 < j) { if (lane == 0) { L.wm()[w] = xj1; L.wx()[w] = 3; } ++w; }
+        if (xj2 >= 0 && xj2 < j) { if (lane == 0) { L.wm()[w] = xj2; L.wx()[w] = 3; } ++w; }
     }
     const int nD = w - ncand;
     if (w > L.maxw) status = kErrListCapacity;
     __syncthreads();
+    PH_END(S, kPhLoads, tph);
     // ---- candidate predictions of the full recomputes of i and j
     double bt0 = 1e300, bt1 = 1e300;
     int bp0 = INT_MAX, bp1 = INT_MAX, bm0 = -1, bm1 = -1, bty0 = -1, bty1 = -1;
     for (int base = 0; base < ncand; base += kW) {
         const int q = base + lane;
         if (q < ncand && q < L.maxw) {
-            const int ent = L.wm[q];
-            const int owner = L.wx[q];
+            const int ent = L.wm()[q];
+            const int owner = L.wx()[q];
             const int m = ent & 0xFFFFFF;
             int stat = (ent >> 24) & 0xFF;
             const Rec rx = owner == 0 ? P.ri : P.rj;
@@ -722,11 +863,12 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
         pj = bm1;
         cj = bty1;
     }
+    PH_END(S, kPhPred, tph);
     // ---- the down set: per unique bead, its I-ops then its J-ops
     if (lane == 0) {
-        L.blist[0] = i / kBlk;
-        L.blist[1] = j >= 0 ? j / kBlk : i / kBlk;
-        L.blist[2] = extra_blk >= 0 ? extra_blk : i / kBlk;
+        L.blist()[0] = i / kBlk;
+        L.blist()[1] = j >= 0 ? j / kBlk : i / kBlk;
+        L.blist()[2] = extra_blk >= 0 ? extra_blk : i / kBlk;
     }
     int nbl = 3;
     for (int base = 0; base < nD; base += kW) {
@@ -734,17 +876,17 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
         bool first = false, is_i = false;
         int l = -1, cI = 0, cJ = 0;
         if (q < nD && ncand + q < L.maxw) {
-            l = L.wm[ncand + q];
+            l = L.wm()[ncand + q];
             first = true;
             for (int r = 0; r < q; ++r)
-                if (L.wm[ncand + r] == l) {
+                if (L.wm()[ncand + r] == l) {
                     first = false;
                     break;
                 }
             if (first)
                 for (int r = 0; r < nD && ncand + r < L.maxw; ++r)
-                    if (L.wm[ncand + r] == l) {
-                        if (L.wx[ncand + r] == 2) ++cI;
+                    if (L.wm()[ncand + r] == l) {
+                        if (L.wx()[ncand + r] == 2) ++cI;
                         else ++cJ;
                     }
             is_i = (l == i);
@@ -764,7 +906,7 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
                 st_p = pi;
                 st_c = ci;
             } else {
-                const Leaf lf = L.leaf[l];
+                const Leaf lf = leaf_get(L, l);
                 st_t = lf.tim;
                 st_p = lf.nptnr;
                 st_c = leaf_ct(lf.meta);
@@ -792,7 +934,7 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
                                        __shfl(xl.w, leader, kW));
             double ft;
             int fp, fc;
-            full_one(S, T, P, lf_l, rlf, bwlf, xlf, tf, T0, ft, fp, fc, status);
+            full_one(S, L, lf_l, rlf, bwlf, xlf, tf, T0, ft, fp, fc, status);
             if (lane == leader) {
                 st_t = ft;
                 st_p = fp;
@@ -815,26 +957,27 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
             lf.tim = st_t;
             lf.nptnr = st_p;
             lf.meta = make_meta(calc_bin(c, st_t, false, status), st_c);
-            L.leaf[l] = lf;
+            leaf_set(L, l, lf);
         }
         const unsigned long long mm = __ballot(modified);
         const int pre = __popcll(mm & ((1ull << lane) - 1ull));
-        if (modified && nbl + pre < kMaxBlist) L.blist[nbl + pre] = l / kBlk;
+        if (modified && nbl + pre < kMaxBlist) L.blist()[nbl + pre] = l / kBlk;
         nbl += __popcll(mm);
         if (nbl > kMaxBlist) nbl = kMaxBlist;
     }
+    PH_END(S, kPhDown, tph);
     if (lane == 0) {
         Leaf a;
         a.tim = ti;
         a.nptnr = pi;
         a.meta = make_meta(calc_bin(c, ti, false, status), ci);
-        L.leaf[i] = a;
+        leaf_set(L, i, a);
         if (j >= 0) {
             Leaf b;
             b.tim = tj;
             b.nptnr = pj;
             b.meta = make_meta(calc_bin(c, tj, false, status), cj);
-            L.leaf[j] = b;
+            leaf_set(L, j, b);
         }
     }
     // repuls_del_b.f once the rescheduling is done
@@ -857,11 +1000,450 @@ __device__ void reschedule(const SysDev& S, const DevTables& T, const Lds& L, Sc
         S.extra[ncai].z = -1;
         S.extra[ncim1].z = -1;
     }
-    status = wave_or(status);
+    status = wave_status(status);
     if (status && !c.status) c.status = status;
     if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
+    PH_END(S, kPhLeaf, tph);
     rescan_block_list(L, N3, nbl);
+    PH_END(S, kPhRescan, tph);
+}
+
+// partial_events.f:16-201 after an event of pair (i, j) (j < 0: ghost of i).
+// One batch of predictions covers the full recomputes of i and j and of the
+// down-set beads whose calendar partner is i or j (speculatively: the
+// recompute happens only if their op sequence reaches it, and is then the
+// same value); each owner's minimum over (time, list position) comes out of
+// LDS atomics.  The remaining single-pair updates run one down bead per
+// lane.  Three dependent rounds of global loads: the rows of i and j; the
+// down beads, the rows of the batched beads and the candidates of i and j;
+// the partners of the batched beads.  Down sets above one wavefront take
+// the generic path.  extra_blk: a further calendar block to rescan (the
+// ghost entry), or -1
+__device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&, const Pair&, int extra_blk,
+                           bool xpulse_del) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+    const Pair& P = *L.pair();
+
+    const int lane = threadIdx.x, cap = S.cap, N3 = S.N + 3;
+    const int i = P.i, j = P.j;
+    const double tf = c.tfalse;
+    const double T0 = c.interval_max + kLt - tf;
+    const int dbase = L.maxw - (2 * cap + 8);  // down-set region of the work list
+    int status = 0;
+    PH_BEGIN(tph);
+    // ---- round 1: list rows of i and j, whole rows with their lengths
+    const int jr = j >= 0 ? j : i;
+    const int nui = S.nup[i], ndi = S.ndn[i];
+    const int nuj = j >= 0 ? S.nup[j] : 0, ndj = j >= 0 ? S.ndn[j] : 0;
+    const int eui = lane < cap ? S.nb_up[(size_t)i * cap + lane] : 0;
+    const int edi = lane < cap ? S.nb_dn[(size_t)i * cap + lane] : 0;
+    const int euj = lane < cap ? S.nb_up[(size_t)jr * cap + lane] : 0;
+    const int edj = lane < cap ? S.nb_dn[(size_t)jr * cap + lane] : 0;
+    const int xi0 = P.xi.x, xi1 = P.xi.y, xi2 = P.xi.z;
+    const int xj0 = P.xj.x, xj1 = P.xj.y, xj2 = P.xj.z;
+    const int kNoStat = 0xFF << 24;
+    // ---- work list: candidates of i (owner 0) and j (owner 1) at 0..
+    int w = 0;
+    if (lane < nui) {
+        L.wm()[lane] = eui;
+        L.wx()[lane] = 0;
+    }
+    w = nui;
+    if (xi0 > i) { if (lane == 0) { L.wm()[w] = xi0 | kNoStat; L.wx()[w] = 0; } ++w; }
+    if (xi1 > i) { if (lane == 0) { L.wm()[w] = xi1 | kNoStat; L.wx()[w] = 0; } ++w; }
+    if (xi2 > i) { if (lane == 0) { L.wm()[w] = xi2 | kNoStat; L.wx()[w] = 0; } ++w; }
+    if (j >= 0) {
+        if (lane < nuj) {
+            L.wm()[w + lane] = euj;
+            L.wx()[w + lane] = 1;
+        }
+        w += nuj;
+        if (xj0 > j) { if (lane == 0) { L.wm()[w] = xj0 | kNoStat; L.wx()[w] = 1; } ++w; }
+        if (xj1 > j) { if (lane == 0) { L.wm()[w] = xj1 | kNoStat; L.wx()[w] = 1; } ++w; }
+        if (xj2 > j) { if (lane == 0) { L.wm()[w] = xj2 | kNoStat; L.wx()[w] = 1; } ++w; }
+    }
+    const int ncand = w;
+    // ---- the down set at dbase.., tag 2 (op with i) / 3 (op with j), in
+    //      reference order: i's down list and lower shoulder partners, then j's
+    int d = 0;
+    if (lane < ndi) {
+        L.wm()[dbase + lane] = edi & 0xFFFFFF;
+        L.wx()[dbase + lane] = 2;
+    }
+    d = ndi;
+    if (xi0 >= 0 && xi0 < i) { if (lane == 0) { L.wm()[dbase + d] = xi0; L.wx()[dbase + d] = 2; } ++d; }
+    if (xi1 >= 0 && xi1 < i) { if (lane == 0) { L.wm()[dbase + d] = xi1; L.wx()[dbase + d] = 2; } ++d; }
+    if (xi2 >= 0 && xi2 < i) { if (lane == 0) { L.wm()[dbase + d] = xi2; L.wx()[dbase + d] = 2; } ++d; }
+    if (j >= 0) {
+        const int l = edj & 0xFFFFFF;
+        const bool keep = lane < ndj && l != i;
+        const unsigned long long bal = __ballot(keep);
+        const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+        if (keep) {
+            L.wm()[dbase + d + pre] = l;
+            L.wx()[dbase + d + pre] = 3;
+        }
+        d += __popcll(bal);
+        if (xj0 >= 0 && xj0 < j) { if (lane == 0) { L.wm()[dbase + d] = xj0; L.wx()[dbase + d] = 3; } ++d; }
+        if (xj1 >= 0 && xj1 < j) { if (lane == 0) { L.wm()[dbase + d] = xj1; L.wx()[dbase + d] = 3; } ++d; }
+        if (xj2 >= 0 && xj2 < j) { if (lane == 0) { L.wm()[dbase + d] = xj2; L.wx()[dbase + d] = 3; } ++d; }
+    }
+    const int nD = d;
+    if (nD > kW) {
+        __syncthreads();
+        reschedule_generic(S, T, L, c, P, extra_blk, xpulse_del);
+        return;
+    }
+    if (lane < 2 + kMaxSpec) {
+        L.slot()[lane].tkey = ~0ull;
+        L.slot()[lane].packed = ~0u;
+    }
+    wave_sync();
+    PH_END(S, kPhRows, tph);
+    // ---- one lane per down-set entry: first occurrence of its bead, the
+    //      bead's op counts, and whether its full recompute is batched
+    int l = -1, cI = 0, cJ = 0;
+    bool first = false;
+    {
+        // every lane compares its bead with each entry, read from lanes
+        const int lv = lane < nD ? L.wm()[dbase + lane] : -1;
+        const int tv = lane < nD ? L.wx()[dbase + lane] : 0;
+        l = lv;
+        first = lane < nD;
+        for (int r = 0; r < nD; ++r) {
+            const int lr = __builtin_amdgcn_readlane(lv, r);
+            const int tr = __builtin_amdgcn_readlane(tv, r);
+            if (lr == lv) {
+                if (r < lane) first = false;
+                if (tr == 2) ++cI;
+                else ++cJ;
+            }
+        }
+        if (!first) {
+            cI = 0;
+            cJ = 0;
+        }
+    }
+    const bool is_i = first && l == i;
+    Leaf lf0;
+    lf0.tim = 0.0;
+    lf0.nptnr = -1;
+    lf0.meta = 0;
+    if (first && !is_i) lf0 = leaf_get(L, l);
+    bool spec = first && !is_i && ((lf0.nptnr == i && cI > 0) || (lf0.nptnr == j && cJ > 0));
+    const unsigned long long smask = __ballot(spec);
+    const int myslot = spec ? __popcll(smask & ((1ull << lane) - 1ull)) : -1;
+    if (myslot >= kMaxSpec) spec = false;
+    const int nspec = min(__popcll(smask), kMaxSpec);
+    if (spec) L.spec()[myslot].l = l;
+    wave_sync();
+    PH_END(S, kPhScan, tph);
+    // ---- round 2: the down beads' own data, the rows of the batched beads
+    //      and the candidates of i and j, all in flight together
+    Rec rl = P.ri;
+    int4 ol = make_int4(-1, -1, -1, -1), xl = make_int4(-1, -1, -1, -1);
+    double2 bwl = make_double2(0.0, 0.0);
+    int nul = 0;
+    if (first) {
+        rl = load_rec(S, P, l);
+        ol = S.ovr[l];
+        bwl = S.bw[l];
+        xl = S.extra[l];
+    }
+    if (spec) nul = S.nup[l];
+    int sent[kMaxSpec];
+#pragma unroll
+    for (int s2 = 0; s2 < kMaxSpec; ++s2) {
+        sent[s2] = 0;
+        if (s2 < nspec && lane < cap) sent[s2] = S.nb_up[(size_t)L.spec()[s2].l * cap + lane];
+    }
+    const int cent = lane < ncand ? L.wm()[lane] : 0;
+    const int cm = cent & 0xFFFFFF;
+    Rec crm = P.ri;
+    int4 com = make_int4(-1, -1, -1, -1);
+    if (lane < ncand) {
+        crm = load_rec(S, P, cm);
+        com = S.ovr[cm];
+    }
+    // the batched beads' pair entries, after the candidates
+    if (spec) {
+        SpecInfo& si = L.spec()[myslot];
+        si.r = rl;
+        si.bw = bwl;
+        si.x = xl;
+        si.nu = nul;
+        si.nx = (xl.x > l ? 1 : 0) + (xl.y > l ? 1 : 0) + (xl.z > l ? 1 : 0);
+    }
+    wave_sync();
+    if (lane == 0) {
+        int off = ncand;
+        for (int s2 = 0; s2 < nspec; ++s2) {
+            SpecInfo& si = L.spec()[s2];
+            const int n = si.nu + si.nx;
+            if (off + n <= dbase) {
+                si.off = off;
+                off += n;
+            } else
+                si.off = -1;  // no room: this bead's recompute runs on its own
+        }
+        L.wd()[0] = off;
+    }
+    wave_sync();
+    const int npairs = L.wd()[0];
+#pragma unroll
+    for (int s2 = 0; s2 < kMaxSpec; ++s2) {
+        if (s2 < nspec) {
+            const int off = L.spec()[s2].off, nu = L.spec()[s2].nu;
+            if (off >= 0) {
+                if (lane < nu) {
+                    L.wm()[off + lane] = sent[s2];
+                    L.wx()[off + lane] = 2 + s2;
+                }
+                if (lane == 0) {
+                    const int4 x = L.spec()[s2].x;
+                    const int ls = L.spec()[s2].l;
+                    int k = off + nu;
+                    if (x.x > ls) { L.wm()[k] = x.x | kNoStat; L.wx()[k] = 2 + s2; ++k; }
+                    if (x.y > ls) { L.wm()[k] = x.y | kNoStat; L.wx()[k] = 2 + s2; ++k; }
+                    if (x.z > ls) { L.wm()[k] = x.z | kNoStat; L.wx()[k] = 2 + s2; ++k; }
+                }
+            }
+        }
+    }
+    wave_sync();
+    PH_END(S, kPhLoads, tph);
+    // ---- the single-pair updates of the down beads up to their first full
+    //      recompute (bead i, reached through extra(j,:), waits for its own
+    //      fresh result)
+    double st_t = lf0.tim;
+    int st_p = lf0.nptnr, st_c = leaf_ct(lf0.meta);
+    bool modified = false, pending = false;
+    int rem = first ? cI + cJ : 0;
+    if (first && !is_i) {
+        while (rem > 0 && !pending) {
+            const bool opi = rem > cJ;
+            const int p = opi ? i : j;
+            --rem;
+            if (st_p != p)
+                single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
+            else
+                pending = true;
+        }
+    }
+    // ---- the prediction batch (round 3: the batched beads' partners)
+    for (int base = 0; base < npairs; base += kW) {
+        const int q = base + lane;
+        if (q < npairs) {
+            const int ent = base == 0 && q < ncand ? cent : L.wm()[q];
+            const int owner = L.wx()[q];
+            const int m = ent & 0xFFFFFF;
+            int stat = (ent >> 24) & 0xFF;
+            int x;
+            Rec rx;
+            double2 bwx;
+            if (owner == 0) {
+                x = i;
+                rx = P.ri;
+                bwx = P.bwi;
+            } else if (owner == 1) {
+                x = j;
+                rx = P.rj;
+                bwx = P.bwj;
+            } else {
+                const SpecInfo& si = L.spec()[owner - 2];
+                x = si.l;
+                rx = si.r;
+                bwx = si.bw;
+            }
+            Rec rm;
+            int4 om;
+            if (base == 0 && q < ncand) {
+                rm = crm;
+                om = com;
+            } else {
+                rm = load_rec(S, P, m);
+                om = S.ovr[m];
+            }
+            if (stat == 0xFF) stat = static_t(rm.topo, rx.topo);
+            const int code = dyn_code(om, x, stat);  // ev_code(m, x)
+            double tij = 1000000000.0;
+            int ty = 0;
+            predict(T, code, rx, rm, dyn_bptnr(rx.dyn) == m, bwx, tf, tij, ty);
+            if (ty == -99) status = kErrBadCode;
+            if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
+            L.wt()[q] = tij;
+            L.wy()[q] = (((ty + 1) & 31) << 16) | (m & 0xFFFF);  // fields masked: position stays in bits 21..31
+            atomicMin(&L.slot()[owner].tkey, t_order(tij));
+        }
+    }
+    wave_sync();
+    for (int base = 0; base < npairs; base += kW) {
+        const int q = base + lane;
+        if (q < npairs) {
+            const int owner = L.wx()[q];
+            if (t_order(L.wt()[q]) == L.slot()[owner].tkey) atomicMin(&L.slot()[owner].packed, ((unsigned)q << 21) | (unsigned)L.wy()[q]);
+        }
+    }
+    wave_sync();
+    PH_END(S, kPhPred, tph);
+    // owner results (full recompute rule: an event only below T0)
+    double ti = T0 + tf, tj = T0 + tf;
+    int pi = -1, ci = -1, pj = -1, cj = -1;
+    {
+        const unsigned a = L.slot()[0].packed;
+        if (a != ~0u) {
+            const double bt = L.wt()[a >> 21];
+            if (bt < T0) {
+                ti = bt + tf;
+                pi = (int)(a & 0xFFFF);
+                ci = (int)((a >> 16) & 31) - 1;
+            }
+        }
+        const unsigned b = L.slot()[1].packed;
+        if (j >= 0 && b != ~0u) {
+            const double bt = L.wt()[b >> 21];
+            if (bt < T0) {
+                tj = bt + tf;
+                pj = (int)(b & 0xFFFF);
+                cj = (int)((b >> 16) & 31) - 1;
+            }
+        }
+    }
+    const double ti0 = ti;
+    const int pi0 = pi, ci0 = ci;
+    if (is_i) {
+        st_t = ti;
+        st_p = pi;
+        st_c = ci;
+    }
+    // ---- the rest of each down bead's ops; full recomputes from the batch,
+    //      others (not foreseen) cooperatively one at a time
+    const bool have_full = spec && L.spec()[myslot].off >= 0;
+    for (int guard = 0; guard < 64 * 9; ++guard) {
+        while (first && (rem > 0 || pending)) {
+            if (pending) {
+                if (is_i) {
+                    st_t = ti0;
+                    st_p = pi0;
+                    st_c = ci0;
+                } else if (have_full) {
+                    const unsigned a = L.slot()[2 + myslot].packed;
+                    st_t = T0 + tf;
+                    st_p = -1;
+                    st_c = -1;
+                    if (a != ~0u) {
+                        const double bt = L.wt()[a >> 21];
+                        if (bt < T0) {
+                            st_t = bt + tf;
+                            st_p = (int)(a & 0xFFFF);
+                            st_c = (int)((a >> 16) & 31) - 1;
+                        }
+                    }
+                } else
+                    break;
+                modified = true;
+                pending = false;
+            } else {
+                const bool opi = rem > cJ;
+                const int p = opi ? i : j;
+                --rem;
+                if (st_p != p)
+                    single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
+                else
+                    pending = true;
+            }
+        }
+        const unsigned long long pm = __ballot(pending);
+        if (pm == 0) break;
+        const int leader = __ffsll((long long)pm) - 1;
+        const int lf_l = __shfl(l, leader, kW);
+        const Rec rlf = shfl_rec(rl, leader);
+        const double2 bwlf = make_double2(__shfl(bwl.x, leader, kW), __shfl(bwl.y, leader, kW));
+        const int4 xlf = make_int4(__shfl(xl.x, leader, kW), __shfl(xl.y, leader, kW), __shfl(xl.z, leader, kW),
+                                   __shfl(xl.w, leader, kW));
+        double ft;
+        int fp, fc;
+        full_one(S, L, lf_l, rlf, bwlf, xlf, tf, T0, ft, fp, fc, status);
+        if (lane == leader) {
+            st_t = ft;
+            st_p = fp;
+            st_c = fc;
+            modified = true;
+            pending = false;
+        }
+    }
+    // bead i's state lives in (ti, pi, ci) and is written with i's leaf
+    const unsigned long long im = __ballot(is_i);
+    if (im) {
+        const int src = __ffsll((long long)im) - 1;
+        ti = __shfl(st_t, src, kW);
+        pi = __shfl(st_p, src, kW);
+        ci = __shfl(st_c, src, kW);
+    }
+    const bool wr = modified && !is_i;
+    if (wr) {
+        Leaf lf;
+        lf.tim = st_t;
+        lf.nptnr = st_p;
+        lf.meta = make_meta(calc_bin(c, st_t, false, status), st_c);
+        leaf_set(L, l, lf);
+    }
+    if (lane == 0) {
+        L.blist()[0] = i / kBlk;
+        L.blist()[1] = j >= 0 ? j / kBlk : i / kBlk;
+        L.blist()[2] = extra_blk >= 0 ? extra_blk : i / kBlk;
+    }
+    const unsigned long long mm = __ballot(wr);
+    const int pre = __popcll(mm & ((1ull << lane) - 1ull));
+    if (wr) L.blist()[3 + pre] = l / kBlk;
+    const int nbl = 3 + __popcll(mm);
+    PH_END(S, kPhDown, tph);
+    if (lane == 0) {
+        Leaf a;
+        a.tim = ti;
+        a.nptnr = pi;
+        a.meta = make_meta(calc_bin(c, ti, false, status), ci);
+        leaf_set(L, i, a);
+        if (j >= 0) {
+            Leaf b;
+            b.tim = tj;
+            b.nptnr = pj;
+            b.meta = make_meta(calc_bin(c, tj, false, status), cj);
+            leaf_set(L, j, b);
+        }
+    }
+    // repuls_del_b.f once the rescheduling is done
+    if (xpulse_del && lane == 0) {
+        const bool iN = dyn_ident(P.ri.dyn) < dyn_ident(P.rj.dyn);
+        const int a = iN ? i : j, b = iN ? j : i;
+        const int La = topo_len(iN ? P.ri.topo : P.rj.topo), Lb = topo_len(iN ? P.rj.topo : P.ri.topo);
+        const int ncim1 = a + La - 1, ncai = a - La, ncaj = b - 2 * Lb, nnjp1 = b - Lb + 1;
+        int4 ea = S.extra[a], eb = S.extra[b];
+        ea.x = -1;
+        ea.y = -1;
+        ea.w = -1;
+        eb.x = -1;
+        eb.y = -1;
+        eb.w = -1;
+        S.extra[a] = ea;
+        S.extra[b] = eb;
+        S.extra[ncaj].z = -1;
+        S.extra[nnjp1].z = -1;
+        S.extra[ncai].z = -1;
+        S.extra[ncim1].z = -1;
+    }
+    status = wave_status(status);
+    if (status && !c.status) c.status = status;
+    if (!L.leaf_lds || xpulse_del) {
+        __threadfence_block();
+        __syncthreads();
+    } else
+        wave_sync();
+    PH_END(S, kPhLeaf, tph);
+    rescan_block_list(L, N3, nbl);
+    PH_END(S, kPhRescan, tph);
 }
 
 // ------------------------------------------------------------------------
@@ -1086,9 +1668,13 @@ __device__ inline bool terminal_t(int t) {
 
 // One pair event: outcome resolution (main.F90:1487-1634), dynamics, H-bond
 // and shoulder bookkeeping (main.F90:641-921), then partial_events.
-__device__ int pair_event(const SysDev& S, const DevTables& T, const Lds& L, Sc& c, int i, int j, int ct) {
+__device__ int pair_event(const SysDev& S, const DevTables&, const Lds& L, Sc&, int i, int j, int ct) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+
     const int lane = threadIdx.x;
     const double tf = c.tfalse;
+    PH_BEGIN(tpe);
     Pair P;
     P.i = i;
     P.j = j;
@@ -1229,18 +1815,23 @@ __device__ int pair_event(const SysDev& S, const DevTables& T, const Lds& L, Sc&
         P.xi = S.extra[i];
         P.xj = S.extra[j];
     }
-    status = wave_or(status);
+    status = wave_status(status);
     if (status && !c.status) c.status = status;
-    if (lane == 0) *L.pair = P;
+    if (lane == 0) *L.pair() = P;
     __syncthreads();
-    reschedule(S, T, L, c, *L.pair, -1, xpulse);
+    PH_END(S, kPhPair, tpe);
+    reschedule(S, T, L, c, *L.pair(), -1, xpulse);
     return ct;
 }
 
 // Andersen ghost collision, main.F90:997-1049
-__device__ int ghost_event(const SysDev& S, const DevTables& T, const Lds& L, Sc& c) {
+__device__ int ghost_event(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+
     const int lane = threadIdx.x;
     const int N = S.N;
+    PH_BEGIN(tg0);
     int i = 0, guard = 0;
     do {
         i = (int)(rng_u01(S.seed, 0, c.rc0++) * N);
@@ -1294,13 +1885,14 @@ __device__ int ghost_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
         g.tim = tg;
         g.nptnr = -2;
         g.meta = make_meta(calc_bin(c, tg, false, status), -2);
-        L.leaf[N] = g;
+        leaf_set(L, N, g);
     }
     if (status && !c.status) c.status = status;
     if (c.tfalse < c.old_tfalse) c.tfalse = c.old_tfalse;
-    if (lane == 0) *L.pair = P;
+    if (lane == 0) *L.pair() = P;
     __syncthreads();
-    reschedule(S, T, L, c, *L.pair, N / kBlk, false);
+    PH_END(S, kPhGhost, tg0);
+    reschedule(S, T, L, c, *L.pair(), N / kBlk, false);
     return i;
 }
 
@@ -1310,9 +1902,18 @@ __device__ int ghost_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
 // Pair (i, j) is listed when its fine cells (cell_add.f binning) are within
 // the n_wrap=2 stencil (|d| <= 2 per axis, periodic) and either its code is
 // a bond/local one (4-12, 17-26) or r^2 <= rlsq(code) (nbor.f:63-128).
-__device__ void rebuild_lists(const SysDev& S, const DevTables& T, Sc& c) {
+// Steps: fine and coarse cell of every bead; stable counting sort of the
+// beads by coarse cell; copies of position, topology and fine cell in that
+// order; then one lane per bead scans the (up to 27) neighbouring coarse
+// cells, all cell ranges requested before the candidates, each candidate
+// read with two independent loads.  Candidates come in a fixed cell order
+// and ascending bead index within a cell, so the lists are deterministic.
+__device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
+    const DevTables& T = tabs();
     const int lane = threadIdx.x;
     const int N = S.N, M = S.M, Mq = S.Mq, NC = S.ncoarse, cap = S.cap;
+    PH_BEGIN(trl);
+#pragma unroll 4
     for (int k = lane; k < N; k += kW) {
         const Rec b = S.rec[k];
         int cx = (int)((b.x + S.half) / S.width);
@@ -1334,30 +1935,36 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, Sc& c) {
     for (int k = lane; k < N; k += kW) atomicAdd(&S.cstart[S.ckey[k]], 1);
     __threadfence_block();
     __syncthreads();
+    PH_END(S, kPhLCells, trl);
+    // exclusive scan of the cell counts
     int carry = 0;
     for (int base = 0; base < NC; base += kW) {
         const int q = base + lane;
         const int v = q < NC ? S.cstart[q] : 0;
         int incl = v;
-        for (int s = 1; s < kW; s <<= 1) {
-            const int o = __shfl_up(incl, s, kW);
-            if (lane >= s) incl += o;
-        }
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, false);
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, false);
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, false);
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, false);
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x142, 0xa, 0xf, false);
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x143, 0xc, 0xf, false);
         if (q < NC) S.cstart[q] = carry + incl - v;
-        carry += __shfl(incl, kW - 1, kW);
+        carry += __builtin_amdgcn_readlane(incl, kW - 1);
     }
     if (lane == 0) S.cstart[NC] = carry;
     __threadfence_block();
     __syncthreads();
-    // stable scatter in bead order
+    PH_END(S, kPhLScan, trl);
+    // stable scatter in bead order: rank among the lower lanes of the same cell
     for (int base = 0; base < N; base += kW) {
         const int k = base + lane;
         const int key = k < N ? S.ckey[k] : -1;
         int rank = 0, cnt = 0;
-        for (int s = 0; s < kW; ++s) {
-            const int o = __shfl(key, s, kW);
+        const int nl = min(N - base, kW);
+        for (int r = 0; r < nl; ++r) {
+            const int o = __builtin_amdgcn_readlane(key, r);
             if (o == key) {
-                if (s < lane) ++rank;
+                if (r < lane) ++rank;
                 ++cnt;
             }
         }
@@ -1371,61 +1978,132 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, Sc& c) {
         __threadfence_block();
         __syncthreads();
     }
+    PH_END(S, kPhLScatter, trl);
+    // positions, topology and fine cells in sorted order
+#pragma unroll 4
+    for (int p2 = lane; p2 < N; p2 += kW) {
+        const int jb = S.csorted[p2];
+        const Rec b = S.rec[jb];
+        S.spos[p2] = make_double4(b.x, b.y, b.z, 0.0);
+        S.sinfo[p2] = make_int4(jb, b.topo, S.cfine[jb], 0);
+    }
+    __threadfence_block();
+    __syncthreads();
+    PH_END(S, kPhLCopy, trl);
     int mx = 0, status = 0;
     const int noff = Mq > 2 ? 3 : Mq;  // distinct coarse offsets {0, +1, -1}
-    for (int i = lane; i < N; i += kW) {
-        const int fi = S.cfine[i];
+    // per lane, its bead's non-empty neighbour cell ranges, [t][lane] in LDS
+    int2* rng = reinterpret_cast<int2*>(smem + L.o_wm);
+    for (int i0 = 0; i0 < N; i0 += kW) {
+        const int i = i0 + lane;
+        const bool act = i < N;
+        const int ii = act ? i : 0;
+        const int fi = S.cfine[ii];
         const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
-        const int k = S.ckey[i];
+        const int k = S.ckey[ii];
         const int qx = k % Mq, qy = (k / Mq) % Mq, qz = k / (Mq * Mq);
-        const Rec bi = S.rec[i];
+        const Rec bi = S.rec[ii];
+        const int4 oi = S.ovr[ii];
+        int nr = 0;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int a = t / 9, bb = (t / 3) % 3, cc = t % 3;
+            if (act && a < noff && bb < noff && cc < noff) {
+                const int oa = a == 2 ? -1 : a, ob = bb == 2 ? -1 : bb, oc = cc == 2 ? -1 : cc;
+                const int nx = (qx + oa + Mq) % Mq, ny = (qy + ob + Mq) % Mq, nz = (qz + oc + Mq) % Mq;
+                const int cell = nx + Mq * (ny + Mq * nz);
+                const int c0 = S.cstart[cell], c1 = S.cstart[cell + 1];
+                if (c1 > c0) {
+                    rng[nr * kW + lane] = make_int2(c0, c1);
+                    ++nr;
+                }
+            }
+        }
+        wave_sync();
+        // candidates in (cell, index) order, four in flight per lane
+        int ri = 0, p = 0, pe = 0;
+        if (nr > 0) {
+            const int2 r0 = rng[lane];
+            p = r0.x;
+            pe = r0.y;
+        }
         int nu = 0, nd = 0;
-        for (int a = 0; a < noff; ++a)
-            for (int bb = 0; bb < noff; ++bb)
-                for (int cc = 0; cc < noff; ++cc) {
-                    const int oa = a == 2 ? -1 : a, ob = bb == 2 ? -1 : bb, oc = cc == 2 ? -1 : cc;
-                    const int nx = (qx + oa + Mq) % Mq, ny = (qy + ob + Mq) % Mq, nz = (qz + oc + Mq) % Mq;
-                    const int cell = nx + Mq * (ny + Mq * nz);
-                    const int s0 = S.cstart[cell], s1 = S.cstart[cell + 1];
-                    for (int p = s0; p < s1; ++p) {
-                        const int jb = S.csorted[p];
-                        if (jb == i) continue;
-                        const int fj = S.cfine[jb];
-                        int dx = (fj & 1023) - ix, dy = ((fj >> 10) & 1023) - iy, dz = (fj >> 20) - iz;
-                        dx = ((dx % M) + M) % M;
-                        dy = ((dy % M) + M) % M;
-                        dz = ((dz % M) + M) % M;
-                        if ((dx > 2 && dx < M - 2) || (dy > 2 && dy < M - 2) || (dz > 2 && dz < M - 2)) continue;
-                        const Rec bj = S.rec[jb];
-                        const int stat = static_t(bj.topo, bi.topo);
-                        const int code = dyn_code(S.ovr[jb], i, stat);  // ev_code(j,i), nbor.f:63
-                        bool take;
-                        if ((code >= 4 && code <= 12) || (code >= 17 && code < 27)) {
-                            take = true;
-                        } else {
-                            double rx = bi.x - bj.x, ry = bi.y - bj.y, rz = bi.z - bj.z;
-                            rx = rx - round(rx);
-                            ry = ry - round(ry);
-                            rz = rz - round(rz);
-                            const double r2 = rx * rx + ry * ry + rz * rz;
-                            take = code >= 1 && code <= 50 && r2 <= T.rlsq[code];
-                        }
-                        if (!take) continue;
-                        if (jb > i) {
-                            if (nu < cap) S.nb_up[(size_t)i * cap + nu] = jb | (stat << 24);
-                            ++nu;
-                        } else {
-                            if (nd < cap) S.nb_dn[(size_t)i * cap + nd] = jb | (stat << 24);
-                            ++nd;
+        while (__ballot(ri < nr)) {
+            int pu[4];
+            bool vu[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                vu[u] = ri < nr;
+                pu[u] = p;
+                if (vu[u]) {
+                    ++p;
+                    if (p >= pe) {
+                        ++ri;
+                        if (ri < nr) {
+                            const int2 r1 = rng[ri * kW + lane];
+                            p = r1.x;
+                            pe = r1.y;
                         }
                     }
                 }
-        if (nu > cap || nd > cap) status = kErrListCapacity;
-        mx = max(mx, max(nu, nd));
-        S.nup[i] = min(nu, cap);
-        S.ndn[i] = min(nd, cap);
+            }
+            int4 inf[4];
+            double4 pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (vu[u]) {
+                    inf[u] = S.sinfo[pu[u]];
+                    pj[u] = S.spos[pu[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (!vu[u]) continue;
+                const int jb = inf[u].x;
+                if (jb == i) continue;
+                const int fj = inf[u].z;
+                int dx = (fj & 1023) - ix, dy = ((fj >> 10) & 1023) - iy, dz = (fj >> 20) - iz;
+                dx = ((dx % M) + M) % M;
+                dy = ((dy % M) + M) % M;
+                dz = ((dz % M) + M) % M;
+                if ((dx > 2 && dx < M - 2) || (dy > 2 && dy < M - 2) || (dz > 2 && dz < M - 2)) continue;
+                const int stat = static_t(inf[u].y, bi.topo);
+                // ev_code(j,i), nbor.f:63: j's overrides exist only towards i's
+                int code = stat;
+                if ((oi.x >= 0 && (oi.x >> 8) == jb) || (oi.y >= 0 && (oi.y >> 8) == jb) ||
+                    (oi.z >= 0 && (oi.z >> 8) == jb) || (oi.w >= 0 && (oi.w >> 8) == jb))
+                    code = dyn_code(S.ovr[jb], i, stat);
+                bool take;
+                if ((code >= 4 && code <= 12) || (code >= 17 && code < 27)) {
+                    take = true;
+                } else {
+                    double rx = bi.x - pj[u].x, ry = bi.y - pj[u].y, rz = bi.z - pj[u].z;
+                    rx = rx - round(rx);
+                    ry = ry - round(ry);
+                    rz = rz - round(rz);
+                    const double r2 = rx * rx + ry * ry + rz * rz;
+                    take = code >= 1 && code <= 50 && r2 <= T.rlsq[code];
+                }
+                if (!take) continue;
+                if (jb > i) {
+                    if (nu < cap) S.nb_up[(size_t)i * cap + nu] = jb | (stat << 24);
+                    ++nu;
+                } else {
+                    if (nd < cap) S.nb_dn[(size_t)i * cap + nd] = jb | (stat << 24);
+                    ++nd;
+                }
+            }
+        }
+        if (act) {
+            if (nu > cap || nd > cap) status = kErrListCapacity;
+            mx = max(mx, max(nu, nd));
+            S.nup[i] = min(nu, cap);
+            S.ndn[i] = min(nd, cap);
+        }
+        wave_sync();
     }
-    status = wave_or(status);
+    PH_END(S, kPhLSearch, trl);
+    status = wave_status(status);
     if (status && !c.status) c.status = status;
     mx = wave_imax(mx);
     if (lane == 0 && mx > S.st->max_nb_seen) S.st->max_nb_seen = mx;
@@ -1434,7 +2112,10 @@ __device__ void rebuild_lists(const SysDev& S, const DevTables& T, Sc& c) {
 }
 
 // events.f:11-127 with tfalse = 0: every bead's next event, then the calendar
-__device__ void events_all(const SysDev& S, const DevTables& T, const Lds& L, Sc& c) {
+__device__ void events_all(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+
     const int lane = threadIdx.x;
     const int N = S.N, cap = S.cap;
     int status = 0;
@@ -1483,21 +2164,24 @@ __device__ void events_all(const SysDev& S, const DevTables& T, const Lds& L, Sc
         lf.tim = ti;
         lf.nptnr = pi;
         lf.meta = make_meta(calc_bin(c, ti, false, status), ci);
-        L.leaf[i] = lf;
+        leaf_set(L, i, lf);
     }
     for (int e = N + lane; e < N + 3; e += kW) {
-        Leaf lf = L.leaf[e];
+        Leaf lf = leaf_get(L, e);
         lf.meta = make_meta(calc_bin(c, lf.tim, false, status), -2);
-        L.leaf[e] = lf;
+        leaf_set(L, e, lf);
     }
-    status = wave_or(status);
+    status = wave_status(status);
     if (status && !c.status) c.status = status;
     if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
     rescan_all(L, N + 3);
 }
 
-__device__ void rebuild(const SysDev& S, const DevTables& T, const Lds& L, Sc& c) {
+__device__ void rebuild(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+
     const int lane = threadIdx.x;
     for (int k = lane; k < S.N; k += kW) {
         Rec b = S.rec[k];
@@ -1509,27 +2193,43 @@ __device__ void rebuild(const SysDev& S, const DevTables& T, const Lds& L, Sc& c
     }
     __threadfence_block();
     __syncthreads();
-    rebuild_lists(S, T, c);
+    PH_BEGIN(trb);
+    rebuild_lists(S, L, c);
+    PH_END(S, kPhLists, trb);
     events_all(S, T, L, c);
+    PH_END(S, kPhEvents, trb);
+    PH_COUNT(S, kPhNRebuild);
 }
 
 // sync pseudo-event, main.F90:1126-1187
-__device__ void sync_event(const SysDev& S, const DevTables& T, const Lds& L, Sc& c) {
+__device__ void sync_event(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
+
     const int lane = threadIdx.x;
     const int N = S.N;
     const double tf = c.tfalse;
+    PH_BEGIN(tsy);
+    PH_COUNT(S, kPhNSync);
     c.t = c.t + tf;
-    for (int e = lane; e < N + 3; e += kW) L.leaf[e].tim = L.leaf[e].tim - tf;
+    for (int e = lane; e < N + 3; e += kW) {
+        Leaf lf = leaf_get(L, e);
+        lf.tim = lf.tim - tf;
+        leaf_set(L, e, lf);
+    }
     c.interval_max = c.interval_max - tf;
     c.tbin_off = c.tbin_off + tf;
     double moved_far = 0.0;
+    // free flight to real positions and the displacement check (displ.f),
+    // four beads per lane in flight at a time
+#pragma unroll 4
     for (int k = lane; k < N; k += kW) {
         Rec b = S.rec[k];
+        const double4 o = S.old_r[k];
         b.x = b.x + b.vx * tf;
         b.y = b.y + b.vy * tf;
         b.z = b.z + b.vz * tf;
         S.rec[k] = b;
-        const double4 o = S.old_r[k];
         const double a = o.x - b.x, bb = o.y - b.y, cc = o.z - b.z;
         const double dis = a * a + bb * bb + cc * cc;
         const double moved = dis / S.hdelr;
@@ -1556,9 +2256,16 @@ __device__ void sync_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
         c.sortsize = c.interval_max / (double)kNumbin;
         c.tbin_off = 0.0;
         c.nupdates++;
+        PH_END(S, kPhSync, tsy);
         rebuild(S, T, L, c);
+        PH_RESET(tsy);
     } else {
-        rescan_all(L, N + 3);
+        // every entry moved by the same -tf: the cached block and superblock
+        // minima are shifted exactly as their leaves were
+        for (int b = lane; b < L.nblk; b += kW)
+            if (L.blk()[b].bc != kEmptyBc) L.blk()[b].t = L.blk()[b].t - tf;
+        if (lane < L.nsup && L.sup()[lane].bc != kEmptyBc) L.sup()[lane].t = L.sup()[lane].t - tf;
+        __syncthreads();
     }
     int status = 0;
     const double t2 = c.interval * 0.999;
@@ -1567,16 +2274,19 @@ __device__ void sync_event(const SysDev& S, const DevTables& T, const Lds& L, Sc
         lf.tim = t2;
         lf.nptnr = -2;
         lf.meta = make_meta(calc_bin(c, t2, true, status), -2);
-        L.leaf[N + 1] = lf;
-        L.blist[0] = (N + 1) / kBlk;
+        leaf_set(L, N + 1, lf);
+        L.blist()[0] = (N + 1) / kBlk;
     }
     if (status && !c.status) c.status = status;
     if (!L.leaf_lds) __threadfence_block();
     __syncthreads();
     rescan_block_list(L, N + 3, 1);
+    PH_END(S, kPhSync, tsy);
 }
 
-__device__ void output_event(const SysDev& S, const Lds& L, Sc& c) {
+__device__ void output_event(const SysDev& S, const Lds& L, Sc&) {
+    Sc& c = *L.sc();
+
     const int lane = threadIdx.x;
     const int N = S.N;
     int status = 0;
@@ -1586,8 +2296,8 @@ __device__ void output_event(const SysDev& S, const Lds& L, Sc& c) {
         lf.tim = t3;
         lf.nptnr = -2;
         lf.meta = make_meta(calc_bin(c, t3, false, status), -2);
-        L.leaf[N + 2] = lf;
-        L.blist[0] = (N + 2) / kBlk;
+        leaf_set(L, N + 2, lf);
+        L.blist()[0] = (N + 2) / kBlk;
     }
     if (status && !c.status) c.status = status;
     if (!L.leaf_lds) __threadfence_block();
@@ -1596,32 +2306,33 @@ __device__ void output_event(const SysDev& S, const Lds& L, Sc& c) {
 }
 
 __device__ Lds make_lds(const SysDev& S, int nblk_max, int maxw, int leaf_cap) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     Lds L;
     const int N3 = S.N + 3;
     L.nblk = (N3 + kBlk - 1) / kBlk;
     L.nsup = (L.nblk + kSup - 1) / kSup;
     L.maxw = maxw;
-    unsigned char* p = smem;
-    L.tab = (DevTables*)p;
-    p += (sizeof(DevTables) + 15) / 16 * 16;
-    L.blk = (BEnt*)p;
-    p += sizeof(BEnt) * nblk_max;
-    L.sup = (BEnt*)p;
-    p += sizeof(BEnt) * kSup;
-    L.wm = (int*)p;
-    p += sizeof(int) * maxw;
-    L.wx = (int*)p;
-    p += sizeof(int) * maxw;
-    L.blist = (int*)p;
-    p += sizeof(int) * kMaxBlist;
-    p = (unsigned char*)(((uintptr_t)p + 15) & ~(uintptr_t)15);
-    L.pair = (Pair*)p;
-    p += (sizeof(Pair) + 15) / 16 * 16;
-    L.sc = (Sc*)p;
-    p += (sizeof(Sc) + 15) / 16 * 16;
+    int p = 0;
+    auto take = [&](int bytes) {
+        const int o = p;
+        p = (p + bytes + 15) / 16 * 16;
+        return o;
+    };
+    take((int)sizeof(DevTables));  // the tables, at offset 0
+    L.o_blk = take((int)sizeof(Key) * nblk_max);
+    L.o_sup = take((int)sizeof(Key) * kSup);
+    L.o_wm = take((int)sizeof(int) * maxw);
+    L.o_wx = take((int)sizeof(int) * maxw);
+    L.o_blist = take((int)sizeof(int) * kMaxBlist);
+    L.o_wd = take((int)sizeof(int) * maxw);
+    L.o_wy = take((int)sizeof(int) * maxw);
+    L.o_wt = take((int)sizeof(double) * maxw);
+    L.o_slot = take((int)sizeof(MinSlot) * (2 + kMaxSpec));
+    L.o_spec = take((int)sizeof(SpecInfo) * kMaxSpec);
+    L.o_pair = take((int)sizeof(Pair));
+    L.o_sc = take((int)sizeof(Sc));
+    L.o_leaf = p;
     L.leaf_lds = N3 <= leaf_cap;
-    L.leaf = L.leaf_lds ? (Leaf*)p : S.leaf;
+    L.gleaf = S.leaf;
     return L;
 }
 
@@ -1682,10 +2393,10 @@ __device__ void copy_tables(const DevTables* src, DevTables* dst) {
 __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, const int* ids, int nblk_max, int maxw) {
     const SysDev& S = sys[ids[blockIdx.x]];
     Lds L = make_lds(S, nblk_max, maxw, 0);
-    copy_tables(S.tab, L.tab);
+    copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
     __syncthreads();
-    const DevTables& T = *L.tab;
-    Sc& c = *L.sc;
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
     if (threadIdx.x == 0) load_sc(S.st, c);
     __syncthreads();
     const int N = S.N;
@@ -1722,12 +2433,12 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
     Lds L = make_lds(S, nblk_max, maxw, leaf_cap);
     const int lane = threadIdx.x;
     const int N = S.N, N3 = N + 3;
-    copy_tables(S.tab, L.tab);
+    copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
     if (L.leaf_lds)
-        for (int e = lane; e < N3; e += kW) L.leaf[e] = S.leaf[e];
+        for (int e = lane; e < N3; e += kW) L.lleaf()[e] = S.leaf[e];
     __syncthreads();
-    const DevTables& T = *L.tab;
-    Sc& c = *L.sc;
+    const DevTables& T = tabs();
+    Sc& c = *L.sc();
     if (lane == 0) {
         load_sc(S.st, c);
         c.target = c.coll + inc[blockIdx.x];
@@ -1736,22 +2447,23 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
     rescan_all(L, N3);
     int log_count = S.st->log_count;
     while (c.coll < c.target && c.status == 0) {
+        PH_BEGIN(tsel);
         // next event: min over the superblock minima of (bin, time, index)
-        BEnt k = lane < L.nsup ? L.sup[lane] : empty_ent();
-        red_key(k, kW);
-        if (k.bin == INT_MAX) {
+        const Key k = key_min_wave(lane < L.nsup ? L.sup()[lane] : empty_key());
+        if (k.bc == kEmptyBc) {
             c.status = kErrBin;
             break;
         }
-        const int e = k.idx;
+        const int e = key_idx(k);
         c.old_tfalse = c.tfalse;
-        c.nbin = k.bin;
+        c.nbin = key_bin(k);
         c.tfalse = k.t;
         c.coll++;
+        PH_END(S, kPhSelect, tsel);
         int jj = -1, ty = 0;
         if (e < N) {
-            jj = k.p;
-            ty = pair_event(S, T, L, c, e, k.p, k.ct);
+            jj = key_p(k);
+            ty = pair_event(S, T, L, c, e, jj, key_ct(k));
             if (ty >= 1 && ty <= 30 && lane == 0) S.st->nevents[ty]++;
         } else if (e == N) {
             jj = ghost_event(S, T, L, c);
@@ -1776,7 +2488,7 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
     }
     __syncthreads();
     if (L.leaf_lds)
-        for (int e2 = lane; e2 < N3; e2 += kW) S.leaf[e2] = L.leaf[e2];
+        for (int e2 = lane; e2 < N3; e2 += kW) S.leaf[e2] = L.lleaf()[e2];
     if (lane == 0) {
         S.st->log_count = log_count;
         store_sc(S.st, c);
@@ -1804,10 +2516,16 @@ __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const
     type[k] = ty;
 }
 
+// also sized for the list rebuild's per-lane cell ranges (27 x 64 int2 over
+// the work-list arrays)
+int engine_maxw(int cap) { return max((4 + kMaxSpec) * (cap + 3), 768); }
+
+static size_t r16(size_t b) { return (b + 15) / 16 * 16; }
 size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap) {
-    size_t b = (sizeof(DevTables) + 15) / 16 * 16 + sizeof(BEnt) * nblk_max + sizeof(BEnt) * kSup +
-               2 * sizeof(int) * maxw + sizeof(int) * kMaxBlist + 16 + (sizeof(Pair) + 15) / 16 * 16 +
-               (sizeof(Sc) + 15) / 16 * 16;
+    size_t b = r16(sizeof(DevTables)) + r16(sizeof(Key) * nblk_max) + r16(sizeof(Key) * kSup) +
+               4 * r16(sizeof(int) * maxw) + r16(sizeof(int) * kMaxBlist) + r16(sizeof(double) * maxw) +
+               r16(sizeof(MinSlot) * (2 + kMaxSpec)) + r16(sizeof(SpecInfo) * kMaxSpec) + r16(sizeof(Pair)) +
+               r16(sizeof(Sc));
     b += sizeof(Leaf) * (size_t)leaf_cap;
     return b;
 }

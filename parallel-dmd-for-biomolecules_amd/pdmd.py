@@ -55,6 +55,7 @@ class Stats(ctypes.Structure):
         ("pad", ctypes.c_int32),
         ("avg_up", ctypes.c_double),
         ("avg_dn", ctypes.c_double),
+        ("phase_cycles", ctypes.c_int64 * 32),
     ]
 
 
