@@ -77,6 +77,66 @@ def cpu_baseline(systems, seeds, per_sys, param_dir, cores):
     return ncoll / busy, len(work), wall, ncoll
 
 
+def serve_run(systems, seeds, args):
+    """Poisson trajectory requests against the continuous-batching server.
+
+    Requests arrive at rate --serve-rate for --serve-seconds; each picks one of
+    the workload's systems, its own thermostat seed, and a log-uniform number
+    of collisions in [--serve-min, --serve-max].  Latency is completion time
+    minus arrival time on the server's clock.
+    """
+    import pdmd
+    rng = np.random.default_rng(7)
+    n_req = max(1, int(rng.poisson(args.serve_rate * args.serve_seconds)))
+    arr = np.sort(rng.uniform(0.0, args.serve_seconds, n_req))
+    pick = rng.integers(0, len(systems), n_req)
+    ncoll = np.exp(rng.uniform(math.log(args.serve_min), math.log(args.serve_max), n_req)).astype(np.int64)
+    eng = pdmd.Engine()
+    srv = pdmd.Server(eng, mem_budget_bytes=int(args.serve_budget_gib * (1 << 30)), quantum=args.serve_quantum,
+                      slice_ms=args.serve_slice_ms)
+    t0 = time.perf_counter()
+    rids = []
+    k = 0
+    steps = 0
+    while True:
+        now = time.perf_counter() - t0
+        while k < n_req and arr[k] <= now:
+            rids.append(srv.submit(systems[pick[k]], int(ncoll[k]), seed=seeds[pick[k]] + 1000003 * (k + 1)))
+            k += 1
+        srv.step()
+        steps += 1
+        q, r, committed = srv.pending()
+        if k == n_req and q == 0 and r == 0:
+            break
+        if q == 0 and r == 0 and k < n_req:
+            time.sleep(max(0.0, arr[k] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    lat, done_coll, failed = [], 0, 0
+    first_sub, last_done = None, 0.0
+    for rid in rids:
+        info, _ = srv.result(rid)
+        if info.state != 2:
+            failed += 1
+            continue
+        lat.append(info.t_done - info.t_submit)
+        done_coll += info.stats.coll
+        first_sub = info.t_submit if first_sub is None else min(first_sub, info.t_submit)
+        last_done = max(last_done, info.t_done)
+    srv.close()
+    eng.close()
+    lat = np.array(lat)
+    span = last_done - (first_sub or 0.0)
+    return {
+        "requests": n_req, "failed": failed, "steps": steps, "wall_s": round(wall, 2),
+        "collisions": int(done_coll), "collisions_per_s": round(done_coll / span, 1) if span > 0 else None,
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4) if lat.size else None,
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4) if lat.size else None,
+        "arrival_rate_per_s": args.serve_rate, "arrival_seconds": args.serve_seconds,
+        "collisions_per_request": [args.serve_min, args.serve_max], "quantum": args.serve_quantum,
+        "slice_ms": args.serve_slice_ms,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,9 +145,17 @@ def main():
     ap.add_argument("--per-step", type=int, default=20000, help="collisions per system per step")
     ap.add_argument("--nsys", type=int, default=64)
     ap.add_argument("--plain-per-sys", type=int, default=0, help="collisions per system in the plain run (0 = per-step)")
-    ap.add_argument("--cpu-per-sys", type=int, default=300000)
+    ap.add_argument("--cpu-per-sys", type=int, default=3000000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window")
+    ap.add_argument("--serve-min", type=float, default=1e4, help="collisions per request, log-uniform lower end")
+    ap.add_argument("--serve-max", type=float, default=3e5, help="collisions per request, log-uniform upper end")
+    ap.add_argument("--serve-quantum", type=int, default=200000, help="most collisions per request per launch")
+    ap.add_argument("--serve-slice-ms", type=float, default=25.0, help="device time per launch (0: event quanta)")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="device memory for resident requests")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (scale for the HBM byte counters)")
     args = ap.parse_args()
@@ -151,6 +219,10 @@ def main():
         pdt = time.perf_counter() - t0
         plain = pps * len(ids) / pdt
 
+    serve = None
+    if not args.no_serve:
+        serve = serve_run(systems, seeds, args)
+
     cpu = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
@@ -184,8 +256,9 @@ def main():
         },
         "plain_collisions_per_s": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(value / plain, 2),
-        "p50_latency_s": None,
-        "p99_latency_s": None,
+        "p50_latency_s": None if serve is None else serve["p50_latency_s"],
+        "p99_latency_s": None if serve is None else serve["p99_latency_s"],
+        "serving": serve,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 3),

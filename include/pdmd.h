@@ -104,6 +104,12 @@ int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int6
  * blocks until the launch has finished. */
 int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* nevents);
 int pdmd_wait(pdmd_ctx* ctx);
+/* The same with a device wall-clock budget: each system stops between
+ * events once budget_ms have passed since the launch began, or at
+ * max_nevents[k] more entries, whichever comes first (pdmd_stats gives the
+ * entries committed).  For request loops over systems of uneven speed. */
+int pdmd_advance_budget_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* max_nevents,
+                              double budget_ms);
 /* Device time of the most recent pdmd_advance* launch (HIP events on the
  * context's stream), in milliseconds. */
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms);
@@ -149,6 +155,37 @@ int pdmd_generate(const char* param_dir, int32_t nchains, const int32_t* chain_l
 int pdmd_generate_seq(const char* param_dir, int32_t nchains, const int32_t* chain_len, const int32_t* restype_in,
                       double density, double tstar, uint64_t seed, int32_t* restype, int32_t* hp, double* sv,
                       double* boxl);
+
+/* ---- serving: continuous batching of trajectory requests (no reference
+ * counterpart; the reference runs one system per MPI job).  Requests queue
+ * on the host and are admitted as device memory and launch slots free up;
+ * each step waits for the launch in flight, retires finished requests,
+ * admits queued ones (first fit in arrival order) and launches the next
+ * quantum of at most `quantum` calendar entries for every resident request,
+ * cut short after slice_ms of device time when slice_ms > 0 (so systems of
+ * uneven speed finish a launch together).  Times are seconds on the
+ * server's clock (steady clock since creation). */
+typedef struct pdmd_server pdmd_server;
+typedef struct {
+    int32_t state;  /* 0 queued, 1 resident, 2 done, 3 failed */
+    int32_t status; /* engine status of a failed request */
+    int64_t ncoll, done;
+    int32_t nbeads, pad;
+    double t_submit, t_admit, t_done;
+    pdmd_stats_t stats; /* final statistics (done or failed) */
+} pdmd_request_info_t;
+
+int pdmd_server_create(pdmd_ctx* ctx, int64_t mem_budget_bytes, int64_t quantum, double slice_ms, int32_t max_resident,
+                       pdmd_server** out);
+int pdmd_server_destroy(pdmd_server* srv);
+/* Queue a request (the description's arrays are copied). */
+int pdmd_server_submit(pdmd_server* srv, const pdmd_system_desc* desc, int64_t ncoll, int64_t* req_id);
+/* One scheduling step; *ncompleted = requests finished by the launch it retired. */
+int pdmd_server_step(pdmd_server* srv, int32_t* ncompleted);
+int pdmd_server_pending(pdmd_server* srv, int32_t* queued, int32_t* resident, int64_t* committed);
+/* Record of a request; sv (6*nbeads) receives its final state once done. */
+int pdmd_server_result(pdmd_server* srv, int64_t req_id, pdmd_request_info_t* info, double* sv);
+int pdmd_server_release(pdmd_server* srv, int64_t req_id);
 
 #ifdef __cplusplus
 }

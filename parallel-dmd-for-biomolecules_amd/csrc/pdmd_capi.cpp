@@ -18,8 +18,15 @@
 
 namespace pdmd {
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+int report_error(const char* msg) {
+    g_err = msg;
+    return -1;
+}
+
+namespace {
 
 int fail(const std::string& msg, int code = -1) {
     g_err = msg;
@@ -81,6 +88,7 @@ struct pdmd_ctx {
     bool timed = false;
     size_t lds_limit = 65536;
     int device = 0;
+    long long wall_khz = 100000;  // wall_clock64 rate
 };
 
 namespace pdmd {
@@ -188,6 +196,13 @@ int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
             (void)hipGetLastError();
             ctx->lds_limit = (size_t)std::max(lds, 32768);
         }
+    }
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
+            ctx->wall_khz = khz;
+        else
+            (void)hipGetLastError();
     }
     *out = ctx.release();
     return 0;
@@ -486,7 +501,27 @@ int pdmd_nbeads(pdmd_ctx* ctx, int32_t id) {
     return H ? H->N : -1;
 }
 
+namespace pdmd {
+namespace {
+int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev, double budget_ms);
+}
+}  // namespace pdmd
+
 int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev) {
+    return advance_launch(ctx, nsys, ids, nev, 0.0);
+}
+
+int pdmd_advance_budget_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* max_nev,
+                              double budget_ms) {
+    if (!(budget_ms > 0.0)) return fail("pdmd_advance_budget_async: budget must be positive");
+    return advance_launch(ctx, nsys, ids, max_nev, budget_ms);
+}
+
+}  // extern "C"
+
+namespace pdmd {
+namespace {
+int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev, double budget_ms) {
     if (!ctx || nsys <= 0 || !ids || !nev) return fail("pdmd_advance: bad arguments");
     if (nsys > 4096) return fail("pdmd_advance: at most 4096 systems per launch");
     std::vector<int> dids(nsys);
@@ -506,12 +541,19 @@ int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const in
     HIPCHK(hipMemcpyAsync(ctx->d_ids.p, dids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_inc.p, inc.data(), sizeof(long long) * nsys, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit, ctx->stream));
+    long long ticks = 0;
+    if (budget_ms > 0.0) ticks = (long long)(budget_ms * (double)ctx->wall_khz);
+    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit,
+                      ctx->stream, ticks));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     // the copies above read pageable memory synchronously before returning
     return 0;
 }
+}  // namespace
+}  // namespace pdmd
+
+extern "C" {
 
 int pdmd_wait(pdmd_ctx* ctx) {
     if (!ctx) return fail("pdmd_wait: null context");

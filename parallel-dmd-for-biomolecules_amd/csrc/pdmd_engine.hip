@@ -2427,8 +2427,10 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
 
 // The event loop: each wavefront commits events of its system until the
 // system's coll counter has advanced by inc[k].
+// A wall-clock budget (ticks of wall_clock64, 0 = none) stops a system early
+// between events, so that one launch over systems of uneven speed ends evenly.
 __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc, int nblk_max,
-                                            int maxw, int leaf_cap) {
+                                            int maxw, int leaf_cap, long long budget) {
     const SysDev& S = sys[ids[blockIdx.x]];
     Lds L = make_lds(S, nblk_max, maxw, leaf_cap);
     const int lane = threadIdx.x;
@@ -2446,7 +2448,9 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
     __syncthreads();
     rescan_all(L, N3);
     int log_count = S.st->log_count;
+    const long long w0 = wall_clock64();
     while (c.coll < c.target && c.status == 0) {
+        if (budget > 0 && wall_clock64() - w0 > budget) break;
         PH_BEGIN(tsel);
         // next event: min over the superblock minima of (bin, time, index)
         const Key k = key_min_wave(lane < L.nsup ? L.sup()[lane] : empty_key());
@@ -2537,14 +2541,14 @@ hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n,
     return hipGetLastError();
 }
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s) {
+                      size_t lds_limit, hipStream_t s, long long budget_ticks) {
     const int nblk = nblk_for(max_n);
     const size_t base = engine_lds_bytes(nblk, maxw, 0);
     int leaf_cap = 0;
     if (lds_limit > base) leaf_cap = (int)((lds_limit - base) / sizeof(Leaf));
     if (leaf_cap > max_n + 3) leaf_cap = max_n + 3;
     const size_t lds = engine_lds_bytes(nblk, maxw, leaf_cap);
-    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap);
+    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap, budget_ticks);
     return hipGetLastError();
 }
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,

@@ -76,4 +76,7 @@ bool generate(const Params& raw, const GenSpec& spec, GenOut& out, std::string& 
 
 uint64_t splitmix64(uint64_t x);
 
+// Sets the text pdmd_last_error() returns on this thread; returns -1.
+int report_error(const char* msg);
+
 }  // namespace pdmd

@@ -59,6 +59,21 @@ class Stats(ctypes.Structure):
     ]
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("ncoll", ctypes.c_int64),
+        ("done", ctypes.c_int64),
+        ("nbeads", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+        ("stats", Stats),
+    ]
+
+
 class Event(ctypes.Structure):
     _fields_ = [
         ("coll", ctypes.c_int64),
@@ -94,6 +109,8 @@ EXPORTS = [
     "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
     "pdmd_download", "pdmd_stats", "pdmd_log_enable", "pdmd_log_read", "pdmd_predict_pairs", "pdmd_observables",
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
+    "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
+    "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async",
 ]
 
 _lib = None
@@ -127,6 +144,15 @@ def lib():
         L.pdmd_observables.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
         L.pdmd_generate.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
+        L.pdmd_server_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                         ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.pdmd_advance_budget_async.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p, ctypes.c_double]
+        L.pdmd_server_destroy.argtypes = [ctypes.c_void_p]
+        L.pdmd_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(SystemDesc), ctypes.c_int64, _i64p]
+        L.pdmd_server_step.argtypes = [ctypes.c_void_p, _i32p]
+        L.pdmd_server_pending.argtypes = [ctypes.c_void_p, _i32p, _i32p, _i64p]
+        L.pdmd_server_result.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(RequestInfo), _f64p]
+        L.pdmd_server_release.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         L.pdmd_pair_codes.argtypes = [ctypes.POINTER(SystemDesc), ctypes.c_int32, _i32p, _i32p, _i32p]
         L.pdmd_generate_seq.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, _i32p, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
@@ -282,3 +308,52 @@ class Engine:
         o = Obs()
         _check(lib().pdmd_observables(self._ctx, sid, ctypes.byref(o)))
         return o
+
+
+class Server:
+    """Continuous-batching request loop on an Engine's context (pdmd_server_*)."""
+
+    def __init__(self, engine, mem_budget_bytes, quantum, slice_ms=0.0, max_resident=0):
+        self.engine = engine
+        self._h = ctypes.c_void_p()
+        _check(lib().pdmd_server_create(engine._ctx, int(mem_budget_bytes), int(quantum), float(slice_ms),
+                                        int(max_resident), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().pdmd_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, system, ncoll, seed, canon=True, species=None, bptnr=None):
+        d, keep = Engine.make_desc(system, seed, canon, bptnr, species)
+        rid = ctypes.c_int64(0)
+        _check(lib().pdmd_server_submit(self._h, ctypes.byref(d), int(ncoll), ctypes.byref(rid)))
+        return rid.value
+
+    def step(self):
+        n = ctypes.c_int32(0)
+        _check(lib().pdmd_server_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def pending(self):
+        q, r, c = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int64(0)
+        _check(lib().pdmd_server_pending(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(c)))
+        return q.value, r.value, c.value
+
+    def result(self, rid, with_state=False):
+        info = RequestInfo()
+        _check(lib().pdmd_server_result(self._h, int(rid), ctypes.byref(info), None))
+        sv = None
+        if with_state and info.state >= 2:
+            sv = np.zeros(6 * info.nbeads)
+            _check(lib().pdmd_server_result(self._h, int(rid), ctypes.byref(info), _p(sv, ctypes.c_double)))
+        return info, sv
+
+    def release(self, rid):
+        return lib().pdmd_server_release(self._h, int(rid))
