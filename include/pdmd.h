@@ -110,6 +110,11 @@ int pdmd_wait(pdmd_ctx* ctx);
  * entries committed).  For request loops over systems of uneven speed. */
 int pdmd_advance_budget_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* max_nevents,
                               double budget_ms);
+/* Blocking advance that stops each system right after its next output
+ * pseudo-event (entry N+3, main.F90:1191-1246), or at max_nevents[k] more
+ * entries: the points where the reference's driver writes observables
+ * (energy.f, radgyr.f, end2end.f) and configurations (config.f). */
+int pdmd_advance_to_output(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* max_nevents);
 /* Device time of the most recent pdmd_advance* launch (HIP events on the
  * context's stream), in milliseconds. */
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms);

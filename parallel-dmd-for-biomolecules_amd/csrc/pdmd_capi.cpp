@@ -503,7 +503,8 @@ int pdmd_nbeads(pdmd_ctx* ctx, int32_t id) {
 
 namespace pdmd {
 namespace {
-int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev, double budget_ms);
+int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev, double budget_ms,
+                   int flags = 0);
 }
 }  // namespace pdmd
 
@@ -517,11 +518,16 @@ int pdmd_advance_budget_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, c
     return advance_launch(ctx, nsys, ids, max_nev, budget_ms);
 }
 
+int pdmd_advance_to_output(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* max_nev) {
+    if (int rc = advance_launch(ctx, nsys, ids, max_nev, 0.0, kStopAtOutput)) return rc;
+    return pdmd_wait(ctx);
+}
+
 }  // extern "C"
 
 namespace pdmd {
 namespace {
-int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev, double budget_ms) {
+int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t* nev, double budget_ms, int flags) {
     if (!ctx || nsys <= 0 || !ids || !nev) return fail("pdmd_advance: bad arguments");
     if (nsys > 4096) return fail("pdmd_advance: at most 4096 systems per launch");
     std::vector<int> dids(nsys);
@@ -544,7 +550,7 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
     long long ticks = 0;
     if (budget_ms > 0.0) ticks = (long long)(budget_ms * (double)ctx->wall_khz);
     HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit,
-                      ctx->stream, ticks));
+                      ctx->stream, ticks, flags));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     // the copies above read pageable memory synchronously before returning

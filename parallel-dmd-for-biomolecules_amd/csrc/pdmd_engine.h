@@ -18,8 +18,9 @@ hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n,
 // Event loop: every listed system runs until its SysState::target.
 // budget_ticks: wall_clock64 ticks after which each system stops between
 // events (0 = run to the event targets)
+constexpr int kStopAtOutput = 1;  // engine_run flag: stop after the next output pseudo-event
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s, long long budget_ticks = 0);
+                      size_t lds_limit, hipStream_t s, long long budget_ticks = 0, int flags = 0);
 // Test hook: collision times of (i, j) pairs (1-based) of system id.
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s);

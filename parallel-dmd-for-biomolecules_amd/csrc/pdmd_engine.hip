@@ -2428,9 +2428,11 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
 // The event loop: each wavefront commits events of its system until the
 // system's coll counter has advanced by inc[k].
 // A wall-clock budget (ticks of wall_clock64, 0 = none) stops a system early
-// between events, so that one launch over systems of uneven speed ends evenly.
+// between events, so that one launch over systems of uneven speed ends evenly;
+// flags & kStopAtOutput stops it right after its next output pseudo-event
+// (where the reference writes observables and configurations).
 __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc, int nblk_max,
-                                            int maxw, int leaf_cap, long long budget) {
+                                            int maxw, int leaf_cap, long long budget, int flags) {
     const SysDev& S = sys[ids[blockIdx.x]];
     Lds L = make_lds(S, nblk_max, maxw, leaf_cap);
     const int lane = threadIdx.x;
@@ -2449,7 +2451,8 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
     rescan_all(L, N3);
     int log_count = S.st->log_count;
     const long long w0 = wall_clock64();
-    while (c.coll < c.target && c.status == 0) {
+    bool stop_out = false;
+    while (c.coll < c.target && c.status == 0 && !stop_out) {
         if (budget > 0 && wall_clock64() - w0 > budget) break;
         PH_BEGIN(tsel);
         // next event: min over the superblock minima of (bin, time, index)
@@ -2475,6 +2478,7 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
             sync_event(S, T, L, c);
         } else {
             output_event(S, L, c);
+            stop_out = (flags & kStopAtOutput) != 0;
         }
         if (S.log && log_count < S.log_cap) {
             if (lane == 0) {
@@ -2541,14 +2545,14 @@ hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n,
     return hipGetLastError();
 }
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s, long long budget_ticks) {
+                      size_t lds_limit, hipStream_t s, long long budget_ticks, int flags) {
     const int nblk = nblk_for(max_n);
     const size_t base = engine_lds_bytes(nblk, maxw, 0);
     int leaf_cap = 0;
     if (lds_limit > base) leaf_cap = (int)((lds_limit - base) / sizeof(Leaf));
     if (leaf_cap > max_n + 3) leaf_cap = max_n + 3;
     const size_t lds = engine_lds_bytes(nblk, maxw, leaf_cap);
-    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap, budget_ticks);
+    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap, budget_ticks, flags);
     return hipGetLastError();
 }
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,

@@ -110,7 +110,7 @@ EXPORTS = [
     "pdmd_download", "pdmd_stats", "pdmd_log_enable", "pdmd_log_read", "pdmd_predict_pairs", "pdmd_observables",
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
-    "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async",
+    "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
 ]
 
 _lib = None
@@ -147,6 +147,7 @@ def lib():
         L.pdmd_server_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
                                          ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
         L.pdmd_advance_budget_async.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p, ctypes.c_double]
+        L.pdmd_advance_to_output.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
         L.pdmd_server_destroy.argtypes = [ctypes.c_void_p]
         L.pdmd_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(SystemDesc), ctypes.c_int64, _i64p]
         L.pdmd_server_step.argtypes = [ctypes.c_void_p, _i32p]
