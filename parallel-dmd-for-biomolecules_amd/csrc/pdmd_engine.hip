@@ -72,6 +72,51 @@ enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLe
     } while (0)
 #endif
 
+// Global-memory access through the generic pointers of SysDev, as global_*
+// operations (a flat operation would also hold back every LDS wait).
+#define GLB __attribute__((address_space(1)))
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ inline T gld(const T* p) {
+    static_assert(sizeof(T) % 16 == 0, "16-byte multiple");
+    u32x4 t[sizeof(T) / 16];
+    const GLB u32x4* q = (const GLB u32x4*)p;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 16); ++k) t[k] = q[k];
+    T r;
+    __builtin_memcpy(&r, t, sizeof(T));
+    return r;
+}
+template <class T>
+__device__ inline void gst(T* p, const T& v) {
+    static_assert(sizeof(T) % 16 == 0, "16-byte multiple");
+    u32x4 t[sizeof(T) / 16];
+    __builtin_memcpy(t, &v, sizeof(T));
+    GLB u32x4* q = (GLB u32x4*)p;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 16); ++k) q[k] = t[k];
+}
+template <class T>
+__device__ inline T gl1(const T* p) {
+    return *(const GLB T*)p;
+}
+template <class T>
+__device__ inline void gs1(T* p, T v) {
+    *(GLB T*)p = v;
+}
+
+// The workgroup's dynamic LDS.  Views into it are kept as offsets and
+// turned into pointers from this array where they are used, so the compiler
+// keeps LDS accesses as ds_* operations (a pointer stored in a struct and
+// passed on would become a flat pointer).
+extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+
+// the scaled tables sit at the start of the LDS, the system's SysDev right
+// after them (field reads become LDS reads in every function)
+constexpr int kSdOff = (int)((sizeof(DevTables) + 15) / 16 * 16);
+__device__ inline const DevTables& tabs() { return *reinterpret_cast<const DevTables*>(smem); }
+__device__ inline const SysDev& sdev() { return *reinterpret_cast<const SysDev*>(smem + kSdOff); }
+
 // ------------------------------------------------------------------------
 // project RNG (counter-based splitmix64) and a logarithm from basic ops
 // ------------------------------------------------------------------------
@@ -171,9 +216,11 @@ __device__ inline int dyn_code(int4 o, int col, int stat) {
 }
 // single-lane writer of ev_code(row, col) = code (keeps an override only
 // where it differs from the static code)
-__device__ void set_code(const SysDev& S, int row, int col, int code, int& status) {
-    int4 o = S.ovr[row];
-    const int stat = static_t(S.rec[row].topo, S.rec[col].topo);
+__device__ void set_code(const SysDev&, int row, int col, int code, int& status) {
+    const SysDev& S = sdev();
+
+    int4 o = gld(S.ovr + (row));
+    const int stat = static_t(gl1(&S.rec[row].topo), gl1(&S.rec[col].topo));
     if (o.x >= 0 && (o.x >> 8) == col) o.x = -1;
     if (o.y >= 0 && (o.y >> 8) == col) o.y = -1;
     if (o.z >= 0 && (o.z >> 8) == col) o.z = -1;
@@ -186,7 +233,7 @@ __device__ void set_code(const SysDev& S, int row, int col, int code, int& statu
         else if (o.w < 0) o.w = v;
         else status = kErrOverrideSlots;
     }
-    S.ovr[row] = o;
+    gst(S.ovr + (row), o);
 }
 
 // ------------------------------------------------------------------------
@@ -220,152 +267,92 @@ __device__ inline void bond_window(const DevTables& T, int code, double2 bw, dou
     }
 }
 // collision time of pair (a, c) with ev_code `code` at false time tf; tij
-// keeps its preset (1e9) and type its value when no event is predicted
+// keeps its preset (1e9) and type its value when no event is predicted.
+// Every predictor of core.f, bond.f, sqwel.f, sqshlder.f and nc_sqwel.f has
+// the same shape: an inner radius^2 A and an outer radius^2 B with
+//   dA = b^2 - v^2 (r^2 - A),  dB = b^2 - v^2 (r^2 - B)
+// and one of four roots: approach to A, (-b - sqrt dA)/v^2; exit through B,
+// (-b + sqrt dB)/v^2; entry through B, (-b - sqrt dB)/v^2; and the stable
+// exit form of bond.f:63, -(r^2 - B)/(sqrt dB + b).  The per-class rules
+// below only choose A, B, the root and the type, so lanes of different
+// classes share one square root and one division, evaluated exactly as
+// each reference routine writes them.
 __device__ void predict(const DevTables& T, int code, const Rec& a, const Rec& c, bool bonded, double2 bwa,
                         double tf, double& tij, int& type) {
     const int ia = dyn_ident(a.dyn), ic = dyn_ident(c.dyn);
     Geo g;
     geo(a, c, tf, g);
     const double b = g.b, r2 = g.r2, v2 = g.v2;
-    if (code <= 3 || (code >= 17 && code <= 26)) {  // core.f:14-40
-        if (b < 0.0) {
-            const double pe = T.ev_param[1][code];
-            double sigsq = T.sigma_sq[ia][ic] * (pe * pe);
-            if (code >= 22) {
-                const double q = T.sqz610[code - 21][ia > ic ? ia : ic];
-                sigsq = sigsq * (q * q);
-            }
-            const double discr = b * b - v2 * (r2 - sigsq);
-            if (discr > 0.0) {
-                tij = (-b - sqrt(discr)) / v2;
-                type = 1;
-            }
+    const bool is_core = code <= 3 || (code >= 17 && code <= 26);
+    const bool is_bond = code >= 4 && code <= 12;
+    const bool is_nc = code == 15, is_well = code == 16, is_shl = code >= 40;
+    if (!(is_core || is_bond || is_nc || is_well || is_shl)) {
+        type = -99;  // codes 13, 14, 27..39: error in ev_code matrix
+        return;
+    }
+    // inner and outer radii squared
+    const double ssq = T.sigma_sq[ia][ic];
+    double A, B = 0.0;
+    if (is_core) {  // core.f:27-31
+        const double pe = T.ev_param[1][code];
+        A = ssq * (pe * pe);
+        if (code >= 22) {
+            const double q = T.sqz610[code - 21][ia > ic ? ia : ic];
+            A = A * (q * q);
         }
-    } else if (code <= 12) {  // bond.f:17-117
+    } else if (is_bond) {  // bond.f:19-31,70-81
         double blmin, blmax;
         bond_window(T, code, bwa, blmin, blmax);
-        if (b < 0.0) {
-            const double d1 = b * b - v2 * (r2 - blmin * blmin);
-            if (d1 > 0.0) {
-                tij = (-b - sqrt(d1)) / v2;
-                type = 2;
-            } else {
-                const double d2 = b * b - v2 * (r2 - blmax * blmax);
-                if (d2 > 0.0) {
-                    tij = (-b + sqrt(d2)) / v2;
-                    type = 3;
-                }
-            }
-        } else {
-            const double d2 = b * b - v2 * (r2 - blmax * blmax);
-            if (d2 > 0.0) {
-                tij = -(r2 - blmax * blmax) / (sqrt(d2) + b);
-                type = 3;
-            }
-        }
-    } else if (code == 15) {  // nc_sqwel.f:19-122
-        const double diff = r2 - T.welldia_sq[ia][ic];
-        if (ia + ic == 5) {
-            if (b < 0.0) {
-                if (diff < 0.0) {
-                    const double cd = b * b - v2 * (r2 - T.sigma_sq[ia][ic]);
-                    if (cd > 0.0) {
-                        tij = (-b - sqrt(cd)) / v2;
-                        type = 1;
-                    } else {
-                        const double wd = b * b - v2 * diff;
-                        tij = (-b + sqrt(wd)) / v2;
-                        type = 16;
-                    }
-                } else {
-                    const double wd = b * b - v2 * diff;
-                    if (wd > 0.0) {
-                        tij = (-b - sqrt(wd)) / v2;
-                        type = 7;
-                    }
-                }
-            } else if (diff < 0.0) {
-                const double wd = b * b - v2 * diff;
-                tij = (-b + sqrt(wd)) / v2;
-                type = 16;
-            }
-        } else if (bonded) {
-            if (b < 0.0) {
-                const double fs = T.sigma_sq[ia][ic] * T.ev_param[1][15] * T.ev_param[1][15];
-                const double fc = b * b - v2 * (r2 - fs);
-                if (fc > 0.0) {
-                    tij = (-b - sqrt(fc)) / v2;
-                    type = 1;
-                } else {
-                    const double wd = b * b - v2 * diff;
-                    tij = (-b + sqrt(wd)) / v2;
-                    type = 8;
-                }
-            } else {
-                const double wd = b * b - v2 * diff;
-                tij = (-b + sqrt(wd)) / v2;
-                type = 8;
-            }
-        } else if (b < 0.0) {
-            const double wd = b * b - v2 * diff;
-            if (wd > 0.0) {
-                tij = (-b - sqrt(wd)) / v2;
-                type = 9;
-            }
-        }
-    } else if (code == 16) {  // sqwel.f:17-63
-        const double diff = r2 - T.welldia_sq[ia][ic];
-        if (b < 0.0) {
-            if (diff < 0.0) {
-                const double cd = b * b - v2 * (r2 - T.sigma_sq[ia][ic]);
-                if (cd > 0.0) {
-                    tij = (-b - sqrt(cd)) / v2;
-                    type = 1;
-                } else {
-                    const double wd = b * b - v2 * diff;
-                    tij = (-b + sqrt(wd)) / v2;
-                    type = 8;
-                }
-            } else {
-                const double wd = b * b - v2 * diff;
-                if (wd > 0.0) {
-                    tij = (-b - sqrt(wd)) / v2;
-                    type = 4;
-                }
-            }
-        } else if (diff < 0.0) {
-            const double wd = b * b - v2 * diff;
-            tij = (-b + sqrt(wd)) / v2;
-            type = 8;
-        }
-    } else if (code >= 40) {  // sqshlder.f:17-62
-        const double diff = r2 - T.shlddia_sq[ia][ic];
-        if (b < 0.0) {
-            if (diff < 0.0) {
-                const double cd = b * b - v2 * (r2 - T.sigma_sq[ia][ic]);
-                if (cd > 0.0) {
-                    tij = (-b - sqrt(cd)) / v2;
-                    type = 1;
-                } else {
-                    const double sd = b * b - v2 * diff;
-                    tij = (-b + sqrt(sd)) / v2;
-                    type = 10;
-                }
-            } else {
-                const double sd = b * b - v2 * diff;
-                if (sd > 0.0) {
-                    tij = (-b - sqrt(sd)) / v2;
-                    type = 12;
-                }
-            }
-        } else if (diff < 0.0) {
-            const double sd = b * b - v2 * diff;
-            tij = (-b + sqrt(sd)) / v2;
-            type = 10;
-        }
+        A = blmin * blmin;
+        B = blmax * blmax;
     } else {
-        type = -99;  // codes 13, 14, 27..39: error in ev_code matrix
+        // nc_sqwel.f / sqwel.f / sqshlder.f: hard core unscaled, except the
+        // bonded N-C pair (core scaled by ev_param(1,15) twice)
+        A = (is_nc && ia + ic != 5 && bonded) ? ssq * T.ev_param[1][15] * T.ev_param[1][15] : ssq;
+        B = is_shl ? T.shlddia_sq[ia][ic] : T.welldia_sq[ia][ic];
     }
+    const double dA = b * b - v2 * (r2 - A);
+    const double diff = r2 - B;
+    const double dB = b * b - v2 * diff;
+    // root: 0 none, 1 approach A, 2 exit B, 3 enter B, 4 stable exit B
+    int root = 0, ty = 0;
+    const bool in = b < 0.0;
+    if (is_core) {
+        if (in && dA > 0.0) { root = 1; ty = 1; }
+    } else if (is_bond) {
+        if (in) {
+            if (dA > 0.0) { root = 1; ty = 2; }
+            else if (dB > 0.0) { root = 2; ty = 3; }
+        } else if (dB > 0.0) { root = 4; ty = 3; }
+    } else if (is_nc && ia + ic != 5) {
+        if (bonded) {  // nc_sqwel.f:76-101: inside the well by construction
+            if (in && dA > 0.0) { root = 1; ty = 1; }
+            else { root = 2; ty = 8; }
+        } else if (in && dB > 0.0) { root = 3; ty = 9; }
+    } else {
+        // H-bond N-C well (types 1/16/7), side-chain well (1/8/4), shoulder (1/10/12)
+        const int ex = is_nc ? 16 : (is_well ? 8 : 10);
+        const int en = is_nc ? 7 : (is_well ? 4 : 12);
+        if (in) {
+            if (diff < 0.0) {
+                if (dA > 0.0) { root = 1; ty = 1; }
+                else { root = 2; ty = ex; }
+            } else if (dB > 0.0) { root = 3; ty = en; }
+        } else if (diff < 0.0) { root = 2; ty = ex; }
+    }
+    if (root == 0) return;
+    const double sq = sqrt(root == 1 ? dA : dB);
+    double num, den = v2;
+    if (root == 1 || root == 3)
+        num = -b - sq;
+    else if (root == 2)
+        num = -b + sq;
+    else {
+        num = -diff;
+        den = sq + b;
+    }
+    tij = num / den;
+    type = ty;
 }
 
 // ------------------------------------------------------------------------
@@ -518,6 +505,7 @@ struct Sc {
     long long coll, numghosts, nupdates, nforced, target;
     unsigned long long rc0, rc1;
     int nbin, status;
+    long long nev[32];  // tally by resolved coltype (main.F90:926)
 };
 
 __device__ inline int calc_bin(const Sc& c, double tim, bool uncond, int& status) {
@@ -561,11 +549,6 @@ __device__ inline unsigned long long t_order(double t) {
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
-// The workgroup's dynamic LDS.  Views into it are kept as offsets and
-// turned into pointers from this array where they are used, so the compiler
-// keeps LDS accesses as ds_* operations (a pointer stored in a struct and
-// passed on would become a flat pointer).
-extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
 
 // LDS views
 struct Lds {
@@ -587,19 +570,22 @@ struct Lds {
     __device__ Sc* sc() const { return reinterpret_cast<Sc*>(smem + o_sc); }
     __device__ Leaf* lleaf() const { return reinterpret_cast<Leaf*>(smem + o_leaf); }
 };
-// the scaled tables sit at the start of the LDS
-__device__ inline const DevTables& tabs() { return *reinterpret_cast<const DevTables*>(smem); }
-__device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : L.gleaf[k]; }
+// the Lds views themselves, after the SysDev copy
+constexpr int kLdOff = kSdOff + (int)((sizeof(SysDev) + 15) / 16 * 16);
+__device__ inline const Lds& ldsv() { return *reinterpret_cast<const Lds*>(smem + kLdOff); }
+__device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : gld(L.gleaf + k); }
 __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
     if (L.leaf_lds)
         L.lleaf()[k] = v;
     else
-        L.gleaf[k] = v;
+        gst(L.gleaf + k, v);
 }
 
 // block minima of the listed blocks (4 blocks per pass, one row of 16 lanes
 // each), then the superblock minima over the touched superblocks
-__device__ void rescan_block_list(const Lds& L, int N3, int nb) {
+__device__ void rescan_block_list(const Lds&, int N3, int nb) {
+    const Lds& L = ldsv();
+
     const int lane = threadIdx.x;
     for (int base = 0; base < nb; base += 4) {
         const int q = base + (lane >> 4), k = lane & 15;
@@ -628,7 +614,9 @@ __device__ void rescan_block_list(const Lds& L, int N3, int nb) {
     }
     wave_sync();
 }
-__device__ void rescan_all(const Lds& L, int N3) {
+__device__ void rescan_all(const Lds&, int N3) {
+    const Lds& L = ldsv();
+
     const int lane = threadIdx.x;
     for (int base = 0; base < L.nblk; base += 4) {
         const int b = base + (lane >> 4), k = lane & 15;
@@ -650,20 +638,26 @@ __device__ void rescan_all(const Lds& L, int N3) {
 // ------------------------------------------------------------------------
 // rescheduling after an event: partial_events.f:16-201
 // ------------------------------------------------------------------------
-__device__ inline Rec load_rec(const SysDev& S, const Pair& P, int m) {
+__device__ inline Rec load_rec(const SysDev&, const Pair& P, int m) {
+    const SysDev& S = sdev();
+
     if (m == P.i) return P.ri;
     if (m == P.j) return P.rj;
-    return S.rec[m];
+    return gld(S.rec + (m));
 }
 
 // Cooperative full recompute of bead l (rl, bwl, xl wave-uniform): min over
 // l's up list and upper shoulder partners (partial_events.f:121-139).
-__device__ void full_one(const SysDev& S, const Lds& L, int l, const Rec& rl, double2 bwl, int4 xl, double tf,
+__device__ void full_one(const SysDev&, const Lds&, int l, const Rec& rl, double2 bwl, int4 xl, double tf,
                          double T0, double& tim, int& np, int& ct, int& status) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     const Pair& P = *L.pair();
     const int lane = threadIdx.x, cap = S.cap;
-    const int nu = S.nup[l];
+    const int nu = gl1(S.nup + (l));
     const int nx = (xl.x > l ? 1 : 0) + (xl.y > l ? 1 : 0) + (xl.z > l ? 1 : 0);
     const int tot = nu + nx;
     double bt = 1e300;
@@ -673,7 +667,7 @@ __device__ void full_one(const SysDev& S, const Lds& L, int l, const Rec& rl, do
         if (p < tot) {
             int m, stat;
             if (p < nu) {
-                const int ent = S.nb_up[(size_t)l * cap + p];
+                const int ent = gl1(S.nb_up + ((size_t)l * cap + p));
                 m = ent & 0xFFFFFF;
                 stat = ent >> 24;
             } else {
@@ -695,7 +689,7 @@ __device__ void full_one(const SysDev& S, const Lds& L, int l, const Rec& rl, do
                 stat = -1;
             }
             const Rec rm = load_rec(S, P, m);
-            const int4 om = S.ovr[m];
+            const int4 om = gld(S.ovr + (m));
             if (stat < 0) stat = static_t(rm.topo, rl.topo);
             const int code = dyn_code(om, l, stat);  // ev_code(m, l), eventredo_up.f:28
             double tij = 1000000000.0;
@@ -743,8 +737,12 @@ __device__ inline void single_update(const DevTables& T, const Rec& rl, int4 ol,
 
 // Generic form for any down-set size: full recomputes one at a time.
 // extra_blk: a further calendar block to rescan (the ghost entry), or -1
-__device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds& L, Sc&, const Pair&,
+__device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, Sc&, const Pair&,
                                    int extra_blk, bool xpulse_del) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
     const Pair& P = *L.pair();
@@ -756,12 +754,12 @@ __device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds&
     int status = 0;
     PH_BEGIN(tph);
     // ---- list rows of i and j, one entry per lane (cap <= 64)
-    const int nui = S.nup[i], ndi = S.ndn[i];
-    const int nuj = j >= 0 ? S.nup[j] : 0, ndj = j >= 0 ? S.ndn[j] : 0;
-    const int eui = lane < nui ? S.nb_up[(size_t)i * cap + lane] : 0;
-    const int edi = lane < ndi ? S.nb_dn[(size_t)i * cap + lane] : 0;
-    const int euj = lane < nuj ? S.nb_up[(size_t)j * cap + lane] : 0;
-    const int edj = lane < ndj ? S.nb_dn[(size_t)j * cap + lane] : 0;
+    const int nui = gl1(S.nup + (i)), ndi = gl1(S.ndn + (i));
+    const int nuj = j >= 0 ? gl1(S.nup + (j)) : 0, ndj = j >= 0 ? gl1(S.ndn + (j)) : 0;
+    const int eui = lane < nui ? gl1(S.nb_up + ((size_t)i * cap + lane)) : 0;
+    const int edi = lane < ndi ? gl1(S.nb_dn + ((size_t)i * cap + lane)) : 0;
+    const int euj = lane < nuj ? gl1(S.nb_up + ((size_t)j * cap + lane)) : 0;
+    const int edj = lane < ndj ? gl1(S.nb_dn + ((size_t)j * cap + lane)) : 0;
     // ---- work list: candidates of i (tag 0) and j (tag 1), then the down
     //      set with its op partner (tag 2: i, tag 3: j), in reference order
     const int xi0 = P.xi.x, xi1 = P.xi.y, xi2 = P.xi.z;
@@ -826,7 +824,7 @@ __device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds&
             const Rec rx = owner == 0 ? P.ri : P.rj;
             const int x = owner == 0 ? i : j;
             const Rec rm = load_rec(S, P, m);
-            const int4 om = S.ovr[m];
+            const int4 om = gld(S.ovr + (m));
             if (stat == 0xFF) stat = static_t(rm.topo, rx.topo);
             const int code = dyn_code(om, x, stat);  // ev_code(m, x)
             double tij = 1000000000.0;
@@ -898,9 +896,9 @@ __device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds&
         int st_p = -1, st_c = -1;
         if (first) {
             rl = load_rec(S, P, l);
-            ol = S.ovr[l];
-            bwl = S.bw[l];
-            xl = S.extra[l];
+            ol = gld(S.ovr + (l));
+            bwl = gld(S.bw + (l));
+            xl = gld(S.extra + (l));
             if (is_i) {  // bead i, reached through extra(j,:): its fresh result
                 st_t = ti;
                 st_p = pi;
@@ -986,19 +984,19 @@ __device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds&
         const int a = iN ? i : j, b = iN ? j : i;
         const int La = topo_len(iN ? P.ri.topo : P.rj.topo), Lb = topo_len(iN ? P.rj.topo : P.ri.topo);
         const int ncim1 = a + La - 1, ncai = a - La, ncaj = b - 2 * Lb, nnjp1 = b - Lb + 1;
-        int4 ea = S.extra[a], eb = S.extra[b];
+        int4 ea = gld(S.extra + (a)), eb = gld(S.extra + (b));
         ea.x = -1;
         ea.y = -1;
         ea.w = -1;
         eb.x = -1;
         eb.y = -1;
         eb.w = -1;
-        S.extra[a] = ea;
-        S.extra[b] = eb;
-        S.extra[ncaj].z = -1;
-        S.extra[nnjp1].z = -1;
-        S.extra[ncai].z = -1;
-        S.extra[ncim1].z = -1;
+        gst(S.extra + (a), ea);
+        gst(S.extra + (b), eb);
+        gs1(&S.extra[ncaj].z, -1);
+        gs1(&S.extra[nnjp1].z, -1);
+        gs1(&S.extra[ncai].z, -1);
+        gs1(&S.extra[ncim1].z, -1);
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
@@ -1020,8 +1018,12 @@ __device__ void reschedule_generic(const SysDev& S, const DevTables&, const Lds&
 // the partners of the batched beads.  Down sets above one wavefront take
 // the generic path.  extra_blk: a further calendar block to rescan (the
 // ghost entry), or -1
-__device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&, const Pair&, int extra_blk,
+__device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, const Pair&, int extra_blk,
                            bool xpulse_del) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
     const Pair& P = *L.pair();
@@ -1035,12 +1037,12 @@ __device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&,
     PH_BEGIN(tph);
     // ---- round 1: list rows of i and j, whole rows with their lengths
     const int jr = j >= 0 ? j : i;
-    const int nui = S.nup[i], ndi = S.ndn[i];
-    const int nuj = j >= 0 ? S.nup[j] : 0, ndj = j >= 0 ? S.ndn[j] : 0;
-    const int eui = lane < cap ? S.nb_up[(size_t)i * cap + lane] : 0;
-    const int edi = lane < cap ? S.nb_dn[(size_t)i * cap + lane] : 0;
-    const int euj = lane < cap ? S.nb_up[(size_t)jr * cap + lane] : 0;
-    const int edj = lane < cap ? S.nb_dn[(size_t)jr * cap + lane] : 0;
+    const int nui = gl1(S.nup + (i)), ndi = gl1(S.ndn + (i));
+    const int nuj = j >= 0 ? gl1(S.nup + (j)) : 0, ndj = j >= 0 ? gl1(S.ndn + (j)) : 0;
+    const int eui = lane < cap ? gl1(S.nb_up + ((size_t)i * cap + lane)) : 0;
+    const int edi = lane < cap ? gl1(S.nb_dn + ((size_t)i * cap + lane)) : 0;
+    const int euj = lane < cap ? gl1(S.nb_up + ((size_t)jr * cap + lane)) : 0;
+    const int edj = lane < cap ? gl1(S.nb_dn + ((size_t)jr * cap + lane)) : 0;
     const int xi0 = P.xi.x, xi1 = P.xi.y, xi2 = P.xi.z;
     const int xj0 = P.xj.x, xj1 = P.xj.y, xj2 = P.xj.z;
     const int kNoStat = 0xFF << 24;
@@ -1148,16 +1150,16 @@ __device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&,
     int nul = 0;
     if (first) {
         rl = load_rec(S, P, l);
-        ol = S.ovr[l];
-        bwl = S.bw[l];
-        xl = S.extra[l];
+        ol = gld(S.ovr + (l));
+        bwl = gld(S.bw + (l));
+        xl = gld(S.extra + (l));
     }
-    if (spec) nul = S.nup[l];
+    if (spec) nul = gl1(S.nup + (l));
     int sent[kMaxSpec];
 #pragma unroll
     for (int s2 = 0; s2 < kMaxSpec; ++s2) {
         sent[s2] = 0;
-        if (s2 < nspec && lane < cap) sent[s2] = S.nb_up[(size_t)L.spec()[s2].l * cap + lane];
+        if (s2 < nspec && lane < cap) sent[s2] = gl1(S.nb_up + ((size_t)L.spec()[s2].l * cap + lane));
     }
     const int cent = lane < ncand ? L.wm()[lane] : 0;
     const int cm = cent & 0xFFFFFF;
@@ -1165,7 +1167,7 @@ __device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&,
     int4 com = make_int4(-1, -1, -1, -1);
     if (lane < ncand) {
         crm = load_rec(S, P, cm);
-        com = S.ovr[cm];
+        com = gld(S.ovr + (cm));
     }
     // the batched beads' pair entries, after the candidates
     if (spec) {
@@ -1264,7 +1266,7 @@ __device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&,
                 om = com;
             } else {
                 rm = load_rec(S, P, m);
-                om = S.ovr[m];
+                om = gld(S.ovr + (m));
             }
             if (stat == 0xFF) stat = static_t(rm.topo, rx.topo);
             const int code = dyn_code(om, x, stat);  // ev_code(m, x)
@@ -1420,19 +1422,19 @@ __device__ void reschedule(const SysDev& S, const DevTables&, const Lds& L, Sc&,
         const int a = iN ? i : j, b = iN ? j : i;
         const int La = topo_len(iN ? P.ri.topo : P.rj.topo), Lb = topo_len(iN ? P.rj.topo : P.ri.topo);
         const int ncim1 = a + La - 1, ncai = a - La, ncaj = b - 2 * Lb, nnjp1 = b - Lb + 1;
-        int4 ea = S.extra[a], eb = S.extra[b];
+        int4 ea = gld(S.extra + (a)), eb = gld(S.extra + (b));
         ea.x = -1;
         ea.y = -1;
         ea.w = -1;
         eb.x = -1;
         eb.y = -1;
         eb.w = -1;
-        S.extra[a] = ea;
-        S.extra[b] = eb;
-        S.extra[ncaj].z = -1;
-        S.extra[nnjp1].z = -1;
-        S.extra[ncai].z = -1;
-        S.extra[ncim1].z = -1;
+        gst(S.extra + (a), ea);
+        gst(S.extra + (b), eb);
+        gs1(&S.extra[ncaj].z, -1);
+        gs1(&S.extra[nnjp1].z, -1);
+        gs1(&S.extra[ncai].z, -1);
+        gs1(&S.extra[ncim1].z, -1);
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
@@ -1601,7 +1603,9 @@ __device__ inline void shoulder_idx(int i, int Li, int j, int Lj, int& ncim1, in
     nnjp1 = j - Lj + 1;
 }
 // repuls_check.f:16-81 (n = the N bead, cb = the C bead)
-__device__ double repuls_check(const SysDev& S, const DevTables& T, const Pair& P, int n, int cb, double tf) {
+__device__ double repuls_check(const SysDev&, const DevTables& T, const Pair& P, int n, int cb, double tf) {
+    const SysDev& S = sdev();
+
     const Rec rn = load_rec(S, P, n), rc = load_rec(S, P, cb);
     int ncim1, ncai, ncaj, nnjp1;
     shoulder_idx(n, topo_len(rn.topo), cb, topo_len(rc.topo), ncim1, ncai, ncaj, nnjp1);
@@ -1611,8 +1615,10 @@ __device__ double repuls_check(const SysDev& S, const DevTables& T, const Pair& 
     return 11.0;
 }
 // repuls_check_3.f:16-102
-__device__ double repuls_check_3(const SysDev& S, const DevTables& T, const Pair& P, int n, int cb, int k,
+__device__ double repuls_check_3(const SysDev&, const DevTables& T, const Pair& P, int n, int cb, int k,
                                  double tf) {
+    const SysDev& S = sdev();
+
     const Rec rn = load_rec(S, P, n), rc = load_rec(S, P, cb);
     int ncim1, ncai, ncaj, nnjp1;
     shoulder_idx(n, topo_len(rn.topo), cb, topo_len(rc.topo), ncim1, ncai, ncaj, nnjp1);
@@ -1624,7 +1630,9 @@ __device__ double repuls_check_3(const SysDev& S, const DevTables& T, const Pair
     return m == 3 ? 10.0 : 11.0;
 }
 // repuls_add.f:14-47 / repuls_del_a.f:14-37 (single lane; n = N bead, cb = C bead)
-__device__ void repuls_add(const SysDev& S, int n, int Ln, int cb, int Lc, int& status) {
+__device__ void repuls_add(const SysDev&, int n, int Ln, int cb, int Lc, int& status) {
+    const SysDev& S = sdev();
+
     int ncim1, ncai, ncaj, nnjp1;
     shoulder_idx(n, Ln, cb, Lc, ncim1, ncai, ncaj, nnjp1);
     set_code(S, ncaj, n, 50, status);
@@ -1635,21 +1643,23 @@ __device__ void repuls_add(const SysDev& S, int n, int Ln, int cb, int Lc, int& 
     set_code(S, cb, ncai, 40, status);
     set_code(S, cb, ncim1, 40, status);
     set_code(S, ncim1, cb, 50, status);
-    int4 en = S.extra[n], ec = S.extra[cb];
+    int4 en = gld(S.extra + (n)), ec = gld(S.extra + (cb));
     en.x = ncaj;
     en.y = nnjp1;
     en.w = cb;
     ec.x = ncai;
     ec.y = ncim1;
     ec.w = n;
-    S.extra[n] = en;
-    S.extra[cb] = ec;
-    S.extra[ncaj].z = n;
-    S.extra[nnjp1].z = n;
-    S.extra[ncai].z = cb;
-    S.extra[ncim1].z = cb;
+    gst(S.extra + (n), en);
+    gst(S.extra + (cb), ec);
+    gs1(&S.extra[ncaj].z, n);
+    gs1(&S.extra[nnjp1].z, n);
+    gs1(&S.extra[ncai].z, cb);
+    gs1(&S.extra[ncim1].z, cb);
 }
-__device__ void repuls_del_a(const SysDev& S, int n, int Ln, int cb, int Lc, int& status) {
+__device__ void repuls_del_a(const SysDev&, int n, int Ln, int cb, int Lc, int& status) {
+    const SysDev& S = sdev();
+
     int ncim1, ncai, ncaj, nnjp1;
     shoulder_idx(n, Ln, cb, Lc, ncim1, ncai, ncaj, nnjp1);
     set_code(S, ncaj, n, 1, status);
@@ -1668,7 +1678,11 @@ __device__ inline bool terminal_t(int t) {
 
 // One pair event: outcome resolution (main.F90:1487-1634), dynamics, H-bond
 // and shoulder bookkeeping (main.F90:641-921), then partial_events.
-__device__ int pair_event(const SysDev& S, const DevTables&, const Lds& L, Sc&, int i, int j, int ct) {
+__device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int i, int j, int ct) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
@@ -1678,13 +1692,13 @@ __device__ int pair_event(const SysDev& S, const DevTables&, const Lds& L, Sc&, 
     Pair P;
     P.i = i;
     P.j = j;
-    P.ri = S.rec[i];
-    P.rj = S.rec[j];
-    P.bwi = S.bw[i];
-    P.bwj = S.bw[j];
-    P.xi = S.extra[i];
-    P.xj = S.extra[j];
-    const int4 oi = S.ovr[i];
+    P.ri = gld(S.rec + (i));
+    P.rj = gld(S.rec + (j));
+    P.bwi = gld(S.bw + (i));
+    P.bwj = gld(S.bw + (j));
+    P.xi = gld(S.extra + (i));
+    P.xj = gld(S.extra + (j));
+    const int4 oi = gld(S.ovr + (i));
     const int evcode = dyn_code(oi, j, static_t(P.ri.topo, P.rj.topo));  // ev_code(i, j)
     const bool ok = !terminal_t(P.ri.topo) && !terminal_t(P.rj.topo);
     const int idi = dyn_ident(P.ri.dyn), idj = dyn_ident(P.rj.dyn);
@@ -1779,7 +1793,7 @@ __device__ int pair_event(const SysDev& S, const DevTables&, const Lds& L, Sc&, 
             if (hbp == i) P.ri.dyn = hdyn;
             else if (hbp == j) P.rj.dyn = hdyn;
             else {
-                if (lane == 0) S.rec[hbp].dyn = hdyn;
+                if (lane == 0) gs1(&S.rec[hbp].dyn, hdyn);
                 fence = true;
             }
         }
@@ -1805,27 +1819,31 @@ __device__ int pair_event(const SysDev& S, const DevTables&, const Lds& L, Sc&, 
         }
     }
     if (lane == 0) {
-        S.rec[i] = P.ri;
-        S.rec[j] = P.rj;
+        gst(S.rec + (i), P.ri);
+        gst(S.rec + (j), P.rj);
     }
     if (fence) {
         // shoulder tables changed: make them visible before the predictions
         __threadfence_block();
         __syncthreads();
-        P.xi = S.extra[i];
-        P.xj = S.extra[j];
+        P.xi = gld(S.extra + (i));
+        P.xj = gld(S.extra + (j));
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
     if (lane == 0) *L.pair() = P;
-    __syncthreads();
+    wave_sync();
     PH_END(S, kPhPair, tpe);
     reschedule(S, T, L, c, *L.pair(), -1, xpulse);
     return ct;
 }
 
 // Andersen ghost collision, main.F90:997-1049
-__device__ int ghost_event(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+__device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
@@ -1840,9 +1858,9 @@ __device__ int ghost_event(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     Pair P;
     P.i = i;
     P.j = -1;
-    P.ri = S.rec[i];
-    P.bwi = S.bw[i];
-    P.xi = S.extra[i];
+    P.ri = gld(S.rec + (i));
+    P.bwi = gld(S.bw + (i));
+    P.xi = gld(S.extra + (i));
     P.rj = P.ri;
     P.bwj = P.bwi;
     P.xj = make_int4(-1, -1, -1, -1);
@@ -1880,7 +1898,7 @@ __device__ int ghost_event(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     P.ri = b;
     P.rj = b;
     if (lane == 0) {
-        S.rec[i] = b;
+        gst(S.rec + (i), b);
         Leaf g;
         g.tim = tg;
         g.nptnr = -2;
@@ -1890,7 +1908,7 @@ __device__ int ghost_event(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     if (status && !c.status) c.status = status;
     if (c.tfalse < c.old_tfalse) c.tfalse = c.old_tfalse;
     if (lane == 0) *L.pair() = P;
-    __syncthreads();
+    wave_sync();
     PH_END(S, kPhGhost, tg0);
     reschedule(S, T, L, c, *L.pair(), N / kBlk, false);
     return i;
@@ -1908,31 +1926,35 @@ __device__ int ghost_event(const SysDev& S, const DevTables&, const Lds& L, Sc&)
 // cells, all cell ranges requested before the candidates, each candidate
 // read with two independent loads.  Candidates come in a fixed cell order
 // and ascending bead index within a cell, so the lists are deterministic.
-__device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
+__device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     const int lane = threadIdx.x;
     const int N = S.N, M = S.M, Mq = S.Mq, NC = S.ncoarse, cap = S.cap;
     PH_BEGIN(trl);
 #pragma unroll 4
     for (int k = lane; k < N; k += kW) {
-        const Rec b = S.rec[k];
+        const Rec b = gld(S.rec + (k));
         int cx = (int)((b.x + S.half) / S.width);
         int cy = (int)((b.y + S.half) / S.width);
         int cz = (int)((b.z + S.half) / S.width);
         cx = ((cx % M) + M) % M;
         cy = ((cy % M) + M) % M;
         cz = ((cz % M) + M) % M;
-        S.cfine[k] = cx | (cy << 10) | (cz << 20);
+        gs1(S.cfine + (k), cx | (cy << 10) | (cz << 20));
         const int qx = min(cx >> 1, Mq - 1), qy = min(cy >> 1, Mq - 1), qz = min(cz >> 1, Mq - 1);
-        S.ckey[k] = qx + Mq * (qy + Mq * qz);
+        gs1(S.ckey + (k), qx + Mq * (qy + Mq * qz));
     }
     for (int q = lane; q < NC; q += kW) {
-        S.cstart[q] = 0;
-        S.crun[q] = 0;
+        gs1(S.cstart + (q), 0);
+        gs1(S.crun + (q), 0);
     }
     __threadfence_block();
     __syncthreads();
-    for (int k = lane; k < N; k += kW) atomicAdd(&S.cstart[S.ckey[k]], 1);
+    for (int k = lane; k < N; k += kW) atomicAdd(&S.cstart[gl1(S.ckey + k)], 1);
     __threadfence_block();
     __syncthreads();
     PH_END(S, kPhLCells, trl);
@@ -1940,7 +1962,7 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
     int carry = 0;
     for (int base = 0; base < NC; base += kW) {
         const int q = base + lane;
-        const int v = q < NC ? S.cstart[q] : 0;
+        const int v = q < NC ? gl1(S.cstart + (q)) : 0;
         int incl = v;
         incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, false);
         incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, false);
@@ -1948,17 +1970,17 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
         incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, false);
         incl += __builtin_amdgcn_update_dpp(0, incl, 0x142, 0xa, 0xf, false);
         incl += __builtin_amdgcn_update_dpp(0, incl, 0x143, 0xc, 0xf, false);
-        if (q < NC) S.cstart[q] = carry + incl - v;
+        if (q < NC) gs1(S.cstart + (q), carry + incl - v);
         carry += __builtin_amdgcn_readlane(incl, kW - 1);
     }
-    if (lane == 0) S.cstart[NC] = carry;
+    if (lane == 0) gs1(S.cstart + (NC), carry);
     __threadfence_block();
     __syncthreads();
     PH_END(S, kPhLScan, trl);
     // stable scatter in bead order: rank among the lower lanes of the same cell
     for (int base = 0; base < N; base += kW) {
         const int k = base + lane;
-        const int key = k < N ? S.ckey[k] : -1;
+        const int key = k < N ? gl1(S.ckey + (k)) : -1;
         int rank = 0, cnt = 0;
         const int nl = min(N - base, kW);
         for (int r = 0; r < nl; ++r) {
@@ -1968,12 +1990,12 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
                 ++cnt;
             }
         }
-        const int basepos = k < N ? S.cstart[key] + S.crun[key] : 0;
+        const int basepos = k < N ? gl1(S.cstart + (key)) + gl1(S.crun + (key)) : 0;
         __threadfence_block();
         __syncthreads();
         if (k < N) {
-            S.csorted[basepos + rank] = k;
-            if (rank == cnt - 1) S.crun[key] += cnt;
+            gs1(S.csorted + (basepos + rank), k);
+            if (rank == cnt - 1) gs1(S.crun + (key), gl1(S.crun + (key)) + cnt);
         }
         __threadfence_block();
         __syncthreads();
@@ -1982,10 +2004,10 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
     // positions, topology and fine cells in sorted order
 #pragma unroll 4
     for (int p2 = lane; p2 < N; p2 += kW) {
-        const int jb = S.csorted[p2];
-        const Rec b = S.rec[jb];
-        S.spos[p2] = make_double4(b.x, b.y, b.z, 0.0);
-        S.sinfo[p2] = make_int4(jb, b.topo, S.cfine[jb], 0);
+        const int jb = gl1(S.csorted + (p2));
+        const Rec b = gld(S.rec + (jb));
+        gst(S.spos + (p2), make_double4(b.x, b.y, b.z, 0.0));
+        gst(S.sinfo + (p2), make_int4(jb, b.topo, gl1(S.cfine + jb), 0));
     }
     __threadfence_block();
     __syncthreads();
@@ -1998,12 +2020,12 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
         const int i = i0 + lane;
         const bool act = i < N;
         const int ii = act ? i : 0;
-        const int fi = S.cfine[ii];
+        const int fi = gl1(S.cfine + (ii));
         const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
-        const int k = S.ckey[ii];
+        const int k = gl1(S.ckey + (ii));
         const int qx = k % Mq, qy = (k / Mq) % Mq, qz = k / (Mq * Mq);
-        const Rec bi = S.rec[ii];
-        const int4 oi = S.ovr[ii];
+        const Rec bi = gld(S.rec + (ii));
+        const int4 oi = gld(S.ovr + (ii));
         int nr = 0;
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
@@ -2012,7 +2034,7 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
                 const int oa = a == 2 ? -1 : a, ob = bb == 2 ? -1 : bb, oc = cc == 2 ? -1 : cc;
                 const int nx = (qx + oa + Mq) % Mq, ny = (qy + ob + Mq) % Mq, nz = (qz + oc + Mq) % Mq;
                 const int cell = nx + Mq * (ny + Mq * nz);
-                const int c0 = S.cstart[cell], c1 = S.cstart[cell + 1];
+                const int c0 = gl1(S.cstart + (cell)), c1 = gl1(S.cstart + (cell + 1));
                 if (c1 > c0) {
                     rng[nr * kW + lane] = make_int2(c0, c1);
                     ++nr;
@@ -2052,8 +2074,8 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (vu[u]) {
-                    inf[u] = S.sinfo[pu[u]];
-                    pj[u] = S.spos[pu[u]];
+                    inf[u] = gld(S.sinfo + (pu[u]));
+                    pj[u] = gld(S.spos + (pu[u]));
                 }
             }
 #pragma unroll
@@ -2072,7 +2094,7 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
                 int code = stat;
                 if ((oi.x >= 0 && (oi.x >> 8) == jb) || (oi.y >= 0 && (oi.y >> 8) == jb) ||
                     (oi.z >= 0 && (oi.z >> 8) == jb) || (oi.w >= 0 && (oi.w >> 8) == jb))
-                    code = dyn_code(S.ovr[jb], i, stat);
+                    code = dyn_code(gld(S.ovr + (jb)), i, stat);
                 bool take;
                 if ((code >= 4 && code <= 12) || (code >= 17 && code < 27)) {
                     take = true;
@@ -2086,10 +2108,10 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
                 }
                 if (!take) continue;
                 if (jb > i) {
-                    if (nu < cap) S.nb_up[(size_t)i * cap + nu] = jb | (stat << 24);
+                    if (nu < cap) gs1(S.nb_up + ((size_t)i * cap + nu), jb | (stat << 24));
                     ++nu;
                 } else {
-                    if (nd < cap) S.nb_dn[(size_t)i * cap + nd] = jb | (stat << 24);
+                    if (nd < cap) gs1(S.nb_dn + ((size_t)i * cap + nd), jb | (stat << 24));
                     ++nd;
                 }
             }
@@ -2097,8 +2119,8 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
         if (act) {
             if (nu > cap || nd > cap) status = kErrListCapacity;
             mx = max(mx, max(nu, nd));
-            S.nup[i] = min(nu, cap);
-            S.ndn[i] = min(nd, cap);
+            gs1(S.nup + (i), min(nu, cap));
+            gs1(S.ndn + (i), min(nd, cap));
         }
         wave_sync();
     }
@@ -2112,7 +2134,11 @@ __device__ void rebuild_lists(const SysDev& S, const Lds& L, Sc& c) {
 }
 
 // events.f:11-127 with tfalse = 0: every bead's next event, then the calendar
-__device__ void events_all(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+__device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
@@ -2121,16 +2147,16 @@ __device__ void events_all(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     int status = 0;
     c.nbin = 1;
     for (int i = lane; i < N; i += kW) {
-        const Rec bi = S.rec[i];
-        const double2 bwi = S.bw[i];
+        const Rec bi = gld(S.rec + (i));
+        const double2 bwi = gld(S.bw + (i));
         double ti = c.interval_max + kLt;
         int pi = -1, ci = -1;
-        const int n = S.nup[i];
+        const int n = gl1(S.nup + (i));
         for (int k = 0; k < n; ++k) {
-            const int ent = S.nb_up[(size_t)i * cap + k];
+            const int ent = gl1(S.nb_up + ((size_t)i * cap + k));
             const int m = ent & 0xFFFFFF;
-            const Rec bm = S.rec[m];
-            const int code = dyn_code(S.ovr[m], i, ent >> 24);
+            const Rec bm = gld(S.rec + (m));
+            const int code = dyn_code(gld(S.ovr + (m)), i, ent >> 24);
             double tij = 1000000000.0;
             int ty = 0;
             predict(T, code, bi, bm, dyn_bptnr(bi.dyn) == m, bwi, 0.0, tij, ty);
@@ -2142,12 +2168,12 @@ __device__ void events_all(const SysDev& S, const DevTables&, const Lds& L, Sc&)
             }
             if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
         }
-        const int4 e = S.extra[i];
+        const int4 e = gld(S.extra + (i));
         for (int k = 0; k < 3; ++k) {
             const int m = k == 0 ? e.x : (k == 1 ? e.y : e.z);
             if (m > i) {
-                const Rec bm = S.rec[m];
-                const int code = dyn_code(S.ovr[m], i, static_t(bm.topo, bi.topo));
+                const Rec bm = gld(S.rec + (m));
+                const int code = dyn_code(gld(S.ovr + (m)), i, static_t(bm.topo, bi.topo));
                 double tij = 1000000000.0;
                 int ty = 0;
                 // events.f:79-83: code 1 -> core, anything else -> shoulder
@@ -2178,18 +2204,22 @@ __device__ void events_all(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     rescan_all(L, N + 3);
 }
 
-__device__ void rebuild(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+__device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
     const int lane = threadIdx.x;
     for (int k = lane; k < S.N; k += kW) {
-        Rec b = S.rec[k];
+        Rec b = gld(S.rec + (k));
         b.x = b.x - round(b.x);
         b.y = b.y - round(b.y);
         b.z = b.z - round(b.z);
-        S.rec[k] = b;
-        S.old_r[k] = make_double4(b.x, b.y, b.z, 0.0);
+        gst(S.rec + (k), b);
+        gst(S.old_r + (k), make_double4(b.x, b.y, b.z, 0.0));
     }
     __threadfence_block();
     __syncthreads();
@@ -2202,7 +2232,11 @@ __device__ void rebuild(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
 }
 
 // sync pseudo-event, main.F90:1126-1187
-__device__ void sync_event(const SysDev& S, const DevTables&, const Lds& L, Sc&) {
+__device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
@@ -2224,12 +2258,12 @@ __device__ void sync_event(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     // four beads per lane in flight at a time
 #pragma unroll 4
     for (int k = lane; k < N; k += kW) {
-        Rec b = S.rec[k];
-        const double4 o = S.old_r[k];
+        Rec b = gld(S.rec + (k));
+        const double4 o = gld(S.old_r + (k));
         b.x = b.x + b.vx * tf;
         b.y = b.y + b.vy * tf;
         b.z = b.z + b.vz * tf;
-        S.rec[k] = b;
+        gst(S.rec + (k), b);
         const double a = o.x - b.x, bb = o.y - b.y, cc = o.z - b.z;
         const double dis = a * a + bb * bb + cc * cc;
         const double moved = dis / S.hdelr;
@@ -2284,7 +2318,11 @@ __device__ void sync_event(const SysDev& S, const DevTables&, const Lds& L, Sc&)
     PH_END(S, kPhSync, tsy);
 }
 
-__device__ void output_event(const SysDev& S, const Lds& L, Sc&) {
+__device__ void output_event(const SysDev&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
     Sc& c = *L.sc();
 
     const int lane = threadIdx.x;
@@ -2305,7 +2343,9 @@ __device__ void output_event(const SysDev& S, const Lds& L, Sc&) {
     rescan_block_list(L, N + 3, 1);
 }
 
-__device__ Lds make_lds(const SysDev& S, int nblk_max, int maxw, int leaf_cap) {
+__device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap) {
+    const SysDev& S = sdev();
+
     Lds L;
     const int N3 = S.N + 3;
     L.nblk = (N3 + kBlk - 1) / kBlk;
@@ -2318,6 +2358,8 @@ __device__ Lds make_lds(const SysDev& S, int nblk_max, int maxw, int leaf_cap) {
         return o;
     };
     take((int)sizeof(DevTables));  // the tables, at offset 0
+    take((int)sizeof(SysDev));     // the system's SysDev, at kSdOff
+    take((int)sizeof(Lds));        // these views, at kLdOff
     L.o_blk = take((int)sizeof(Key) * nblk_max);
     L.o_sup = take((int)sizeof(Key) * kSup);
     L.o_wm = take((int)sizeof(int) * maxw);
@@ -2355,6 +2397,7 @@ __device__ inline void load_sc(const SysState* st, Sc& c) {
     c.rc1 = st->rng_ctr[1];
     c.nbin = st->nbin;
     c.status = st->status;
+    for (int k = 0; k < 32; ++k) c.nev[k] = st->nevents[k];
 }
 __device__ inline void store_sc(SysState* st, const Sc& c) {
     st->tfalse = c.tfalse;
@@ -2375,6 +2418,7 @@ __device__ inline void store_sc(SysState* st, const Sc& c) {
     st->rng_ctr[1] = c.rc1;
     st->nbin = c.nbin;
     st->status = c.status;
+    for (int k = 0; k < 32; ++k) st->nevents[k] = c.nev[k];
 }
 __device__ void copy_tables(const DevTables* src, DevTables* dst) {
     const double* s = (const double*)src;
@@ -2391,8 +2435,12 @@ __device__ void copy_tables(const DevTables* src, DevTables* dst) {
 // Start-up of one system per wavefront: wrap, lists, pseudo-event seeds and
 // events() (main.F90:205-424 after the host-side H-bond restore).
 __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, const int* ids, int nblk_max, int maxw) {
-    const SysDev& S = sys[ids[blockIdx.x]];
-    Lds L = make_lds(S, nblk_max, maxw, 0);
+    if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
+    __syncthreads();
+    const SysDev& S = sdev();
+    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, nblk_max, maxw, 0);
+    __syncthreads();
+    const Lds& L = ldsv();
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
     __syncthreads();
     const DevTables& T = tabs();
@@ -2415,9 +2463,9 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
         s3.tim = 3.3 / (sqrt(S.setemp)) + 5;
         g.nptnr = s2.nptnr = s3.nptnr = -2;
         g.meta = s2.meta = s3.meta = make_meta(-1, -2);
-        S.leaf[N] = g;
-        S.leaf[N + 1] = s2;
-        S.leaf[N + 2] = s3;
+        gst(S.leaf + (N), g);
+        gst(S.leaf + (N + 1), s2);
+        gst(S.leaf + (N + 2), s3);
     }
     __threadfence_block();
     __syncthreads();
@@ -2433,13 +2481,17 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
 // (where the reference writes observables and configurations).
 __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc, int nblk_max,
                                             int maxw, int leaf_cap, long long budget, int flags) {
-    const SysDev& S = sys[ids[blockIdx.x]];
-    Lds L = make_lds(S, nblk_max, maxw, leaf_cap);
+    if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
+    __syncthreads();
+    const SysDev& S = sdev();
+    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, nblk_max, maxw, leaf_cap);
+    __syncthreads();
+    const Lds& L = ldsv();
     const int lane = threadIdx.x;
     const int N = S.N, N3 = N + 3;
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
     if (L.leaf_lds)
-        for (int e = lane; e < N3; e += kW) L.lleaf()[e] = S.leaf[e];
+        for (int e = lane; e < N3; e += kW) L.lleaf()[e] = gld(S.leaf + (e));
     __syncthreads();
     const DevTables& T = tabs();
     Sc& c = *L.sc();
@@ -2471,7 +2523,7 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
         if (e < N) {
             jj = key_p(k);
             ty = pair_event(S, T, L, c, e, jj, key_ct(k));
-            if (ty >= 1 && ty <= 30 && lane == 0) S.st->nevents[ty]++;
+            if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
         } else if (e == N) {
             jj = ghost_event(S, T, L, c);
         } else if (e == N + 1) {
@@ -2489,14 +2541,14 @@ __global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, cons
                 r.type = ty;
                 r.pad = 0;
                 r.t = k.t;
-                S.log[log_count] = r;
+                gst(S.log + (log_count), r);
             }
             ++log_count;
         }
     }
     __syncthreads();
     if (L.leaf_lds)
-        for (int e2 = lane; e2 < N3; e2 += kW) S.leaf[e2] = L.lleaf()[e2];
+        for (int e2 = lane; e2 < N3; e2 += kW) gst(S.leaf + (e2), L.lleaf()[e2]);
     if (lane == 0) {
         S.st->log_count = log_count;
         store_sc(S.st, c);
@@ -2515,11 +2567,11 @@ __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const
         type[k] = -100;
         return;
     }
-    const Rec ri = S.rec[i], rj = S.rec[j];
-    const int code = dyn_code(S.ovr[j], i, static_t(rj.topo, ri.topo));
+    const Rec ri = gld(S.rec + (i)), rj = gld(S.rec + (j));
+    const int code = dyn_code(gld(S.ovr + (j)), i, static_t(rj.topo, ri.topo));
     double t = 1000000000.0;
     int ty = 0;
-    predict(*S.tab, code, ri, rj, dyn_bptnr(ri.dyn) == j, S.bw[i], S.st->tfalse, t, ty);
+    predict(*S.tab, code, ri, rj, dyn_bptnr(ri.dyn) == j, gld(S.bw + (i)), S.st->tfalse, t, ty);
     tij[k] = t;
     type[k] = ty;
 }
@@ -2530,7 +2582,7 @@ int engine_maxw(int cap) { return max((4 + kMaxSpec) * (cap + 3), 768); }
 
 static size_t r16(size_t b) { return (b + 15) / 16 * 16; }
 size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap) {
-    size_t b = r16(sizeof(DevTables)) + r16(sizeof(Key) * nblk_max) + r16(sizeof(Key) * kSup) +
+    size_t b = r16(sizeof(DevTables)) + r16(sizeof(SysDev)) + r16(sizeof(Lds)) + r16(sizeof(Key) * nblk_max) + r16(sizeof(Key) * kSup) +
                4 * r16(sizeof(int) * maxw) + r16(sizeof(int) * kMaxBlist) + r16(sizeof(double) * maxw) +
                r16(sizeof(MinSlot) * (2 + kMaxSpec)) + r16(sizeof(SpecInfo) * kMaxSpec) + r16(sizeof(Pair)) +
                r16(sizeof(Sc));
