@@ -542,6 +542,13 @@ struct SpecInfo {
     int4 x;
     int l, nu, nx, off;
 };
+// a down-set bead's own data, read by the lanes that predict its single-pair
+// updates inside the batch
+struct DownInfo {
+    Rec r;
+    int4 o;
+    double2 bw;
+};
 // time as an unsigned key with the order of the doubles (-0 as +0)
 __device__ inline unsigned long long t_order(double t) {
     if (t == 0.0) t = 0.0;
@@ -552,7 +559,7 @@ __device__ inline unsigned long long t_order(double t) {
 
 // LDS views
 struct Lds {
-    int o_blk, o_sup, o_wm, o_wx, o_blist, o_pair, o_sc, o_leaf, o_wd, o_wt, o_wy, o_slot, o_spec;
+    int o_blk, o_sup, o_wm, o_wx, o_blist, o_pair, o_sc, o_leaf, o_wd, o_wt, o_wy, o_slot, o_spec, o_down;
     Leaf* gleaf;  // calendar leaves in HBM when they do not fit in LDS
     int nblk, nsup, maxw;
     bool leaf_lds;
@@ -566,6 +573,7 @@ struct Lds {
     __device__ int* blist() const { return reinterpret_cast<int*>(smem + o_blist); }  // blocks to rescan
     __device__ MinSlot* slot() const { return reinterpret_cast<MinSlot*>(smem + o_slot); }
     __device__ SpecInfo* spec() const { return reinterpret_cast<SpecInfo*>(smem + o_spec); }
+    __device__ DownInfo* down() const { return reinterpret_cast<DownInfo*>(smem + o_down); }
     __device__ Pair* pair() const { return reinterpret_cast<Pair*>(smem + o_pair); }
     __device__ Sc* sc() const { return reinterpret_cast<Sc*>(smem + o_sc); }
     __device__ Leaf* lleaf() const { return reinterpret_cast<Leaf*>(smem + o_leaf); }
@@ -1169,6 +1177,13 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         crm = load_rec(S, P, cm);
         com = gld(S.ovr + (cm));
     }
+    // the down beads' own data for their single-pair predictions
+    if (first && cI + cJ > 0) {
+        DownInfo& di = L.down()[lane];
+        di.r = rl;
+        di.o = ol;
+        di.bw = bwl;
+    }
     // the batched beads' pair entries, after the candidates
     if (spec) {
         SpecInfo& si = L.spec()[myslot];
@@ -1214,70 +1229,90 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             }
         }
     }
-    wave_sync();
-    PH_END(S, kPhLoads, tph);
-    // ---- the single-pair updates of the down beads up to their first full
-    //      recompute (bead i, reached through extra(j,:), waits for its own
-    //      fresh result)
-    double st_t = lf0.tim;
-    int st_p = lf0.nptnr, st_c = leaf_ct(lf0.meta);
-    bool modified = false, pending = false;
-    int rem = first ? cI + cJ : 0;
-    if (first && !is_i) {
-        while (rem > 0 && !pending) {
-            const bool opi = rem > cJ;
-            const int p = opi ? i : j;
-            --rem;
-            if (st_p != p)
-                single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
-            else
-                pending = true;
+    // ---- the single-pair predictions (l, i) and (l, j) of each down bead,
+    //      after the pair entries: each is the same for every op of l with
+    //      that bead, so one per (bead, partner) serves the whole op sequence
+    const bool wantI = first && cI > 0, wantJ = first && cJ > 0;
+    const unsigned long long mI = __ballot(wantI), mJ = __ballot(wantJ);
+    const int nI = __popcll(mI), nsing = nI + __popcll(mJ);
+    const bool sing_fit = npairs + nsing <= L.maxw;
+    int sqI = -1, sqJ = -1;
+    if (sing_fit) {
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (wantI) {
+            sqI = npairs + __popcll(mI & below);
+            L.wm()[sqI] = lane;
+        }
+        if (wantJ) {
+            sqJ = npairs + nI + __popcll(mJ & below);
+            L.wm()[sqJ] = lane | 0x100;
         }
     }
-    // ---- the prediction batch (round 3: the batched beads' partners)
-    for (int base = 0; base < npairs; base += kW) {
+    const int ntot = sing_fit ? npairs + nsing : npairs;
+    wave_sync();
+    PH_END(S, kPhLoads, tph);
+    // ---- the prediction batch (round 3: the batched beads' partners, then
+    //      the single-pair predictions)
+    for (int base = 0; base < ntot; base += kW) {
         const int q = base + lane;
-        if (q < npairs) {
+        if (q < ntot) {
+            const bool single = q >= npairs;
             const int ent = base == 0 && q < ncand ? cent : L.wm()[q];
-            const int owner = L.wx()[q];
-            const int m = ent & 0xFFFFFF;
-            int stat = (ent >> 24) & 0xFF;
-            int x;
-            Rec rx;
-            double2 bwx;
-            if (owner == 0) {
-                x = i;
-                rx = P.ri;
-                bwx = P.bwi;
-            } else if (owner == 1) {
-                x = j;
-                rx = P.rj;
-                bwx = P.bwj;
+            const int owner = single ? 0 : L.wx()[q];
+            int m, stat, xc;
+            Rec ra, rb;
+            int4 oc;
+            double2 bwa;
+            if (single) {
+                // eventredo_down.f: ev_code(l, p) with l's record first
+                const DownInfo& di = L.down()[ent & 0xFF];
+                const bool oj = (ent & 0x100) != 0;
+                m = oj ? j : i;
+                ra = di.r;
+                rb = oj ? P.rj : P.ri;
+                oc = di.o;
+                bwa = di.bw;
+                xc = m;
+                stat = static_t(ra.topo, rb.topo);
             } else {
-                const SpecInfo& si = L.spec()[owner - 2];
-                x = si.l;
-                rx = si.r;
-                bwx = si.bw;
+                m = ent & 0xFFFFFF;
+                stat = (ent >> 24) & 0xFF;
+                if (owner == 0) {
+                    xc = i;
+                    ra = P.ri;
+                    bwa = P.bwi;
+                } else if (owner == 1) {
+                    xc = j;
+                    ra = P.rj;
+                    bwa = P.bwj;
+                } else {
+                    const SpecInfo& si = L.spec()[owner - 2];
+                    xc = si.l;
+                    ra = si.r;
+                    bwa = si.bw;
+                }
+                if (base == 0 && q < ncand) {
+                    rb = crm;
+                    oc = com;
+                } else {
+                    rb = load_rec(S, P, m);
+                    oc = gld(S.ovr + (m));
+                }
+                if (stat == 0xFF) stat = static_t(rb.topo, ra.topo);
             }
-            Rec rm;
-            int4 om;
-            if (base == 0 && q < ncand) {
-                rm = crm;
-                om = com;
-            } else {
-                rm = load_rec(S, P, m);
-                om = gld(S.ovr + (m));
-            }
-            if (stat == 0xFF) stat = static_t(rm.topo, rx.topo);
-            const int code = dyn_code(om, x, stat);  // ev_code(m, x)
+            const int code = dyn_code(oc, xc, stat);  // ev_code(m, x), or ev_code(l, p) for a single
             double tij = 1000000000.0;
             int ty = 0;
-            predict(T, code, rx, rm, dyn_bptnr(rx.dyn) == m, bwx, tf, tij, ty);
-            if (ty == -99) status = kErrBadCode;
-            if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
+            predict(T, code, ra, rb, dyn_bptnr(ra.dyn) == m, bwa, tf, tij, ty);
             L.wt()[q] = tij;
-            L.wy()[q] = (((ty + 1) & 31) << 16) | (m & 0xFFFF);  // fields masked: position stays in bits 21..31
-            atomicMin(&L.slot()[owner].tkey, t_order(tij));
+            if (single) {
+                L.wy()[q] = ty;  // checked where the op sequence uses it
+            } else {
+                if (ty == -99) status = kErrBadCode;
+                if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
+                L.wy()[q] = (((ty + 1) & 31) << 16) | (m & 0xFFFF);  // fields masked: position stays in bits 21..31
+                atomicMin(&L.slot()[owner].tkey, t_order(tij));
+            }
         }
     }
     wave_sync();
@@ -1315,6 +1350,10 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     }
     const double ti0 = ti;
     const int pi0 = pi, ci0 = ci;
+    double st_t = lf0.tim;
+    int st_p = lf0.nptnr, st_c = leaf_ct(lf0.meta);
+    bool modified = false, pending = false;
+    int rem = first ? cI + cJ : 0;
     if (is_i) {
         st_t = ti;
         st_p = pi;
@@ -1350,11 +1389,25 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             } else {
                 const bool opi = rem > cJ;
                 const int p = opi ? i : j;
+                const int sq = opi ? sqI : sqJ;
                 --rem;
-                if (st_p != p)
-                    single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
-                else
+                if (st_p == p)
                     pending = true;
+                else if (sq >= 0) {
+                    // eventredo_down.f from the batch's prediction
+                    double tij = L.wt()[sq];
+                    const int ty = L.wy()[sq];
+                    if (ty == -99) status = kErrBadCode;
+                    if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
+                    tij = tij + tf;
+                    if (tij < st_t) {
+                        st_t = tij;
+                        st_p = p;
+                        st_c = ty;
+                        modified = true;
+                    }
+                } else
+                    single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
             }
         }
         const unsigned long long pm = __ballot(pending);
@@ -2370,6 +2423,7 @@ __device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap) {
     L.o_wt = take((int)sizeof(double) * maxw);
     L.o_slot = take((int)sizeof(MinSlot) * (2 + kMaxSpec));
     L.o_spec = take((int)sizeof(SpecInfo) * kMaxSpec);
+    L.o_down = take((int)sizeof(DownInfo) * kW);
     L.o_pair = take((int)sizeof(Pair));
     L.o_sc = take((int)sizeof(Sc));
     L.o_leaf = p;
@@ -2584,7 +2638,8 @@ static size_t r16(size_t b) { return (b + 15) / 16 * 16; }
 size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap) {
     size_t b = r16(sizeof(DevTables)) + r16(sizeof(SysDev)) + r16(sizeof(Lds)) + r16(sizeof(Key) * nblk_max) + r16(sizeof(Key) * kSup) +
                4 * r16(sizeof(int) * maxw) + r16(sizeof(int) * kMaxBlist) + r16(sizeof(double) * maxw) +
-               r16(sizeof(MinSlot) * (2 + kMaxSpec)) + r16(sizeof(SpecInfo) * kMaxSpec) + r16(sizeof(Pair)) +
+               r16(sizeof(MinSlot) * (2 + kMaxSpec)) + r16(sizeof(SpecInfo) * kMaxSpec) +
+               r16(sizeof(DownInfo) * kW) + r16(sizeof(Pair)) +
                r16(sizeof(Sc));
     b += sizeof(Leaf) * (size_t)leaf_cap;
     return b;
