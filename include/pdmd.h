@@ -115,6 +115,16 @@ int pdmd_advance_budget_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_id
  * entries: the points where the reference's driver writes observables
  * (energy.f, radgyr.f, end2end.f) and configurations (config.f). */
 int pdmd_advance_to_output(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* max_nevents);
+/* Wavefronts per system for the context's later advances (1 to 8; larger
+ * values are lowered to what the device's LDS holds).  With W > 1 every
+ * round of a system's event loop pops up to W calendar entries and runs
+ * the pair events among them side by side, one per wavefront; the round
+ * commits the prefix that the serial order would have run identically
+ * (disjoint footprints, no new calendar entry ahead of a later popped one),
+ * so results are the same for every W.  No counterpart in the reference,
+ * whose master/worker speculation (main.F90:484-1258) this replaces.
+ * Default 1. */
+int pdmd_set_waves(pdmd_ctx* ctx, int32_t waves_per_system);
 /* Device time of the most recent pdmd_advance* launch (HIP events on the
  * context's stream), in milliseconds. */
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms);

@@ -67,6 +67,7 @@ struct HostSys {
     DevBuf<Leaf> leaf;
     DevBuf<double4> old_r, spos;
     DevBuf<int4> sinfo;
+    DevBuf<unsigned> mark;
     DevBuf<SysState> st;
     DevBuf<LogRec> log;
     SysDev dev{};
@@ -89,6 +90,7 @@ struct pdmd_ctx {
     size_t lds_limit = 65536;
     int device = 0;
     long long wall_khz = 100000;  // wall_clock64 rate
+    int waves = 1;                // wavefronts per system
 };
 
 namespace pdmd {
@@ -232,6 +234,7 @@ int64_t pdmd_system_bytes(const pdmd_system_desc* d) {
                 2 * N * cap * 4 + 2 * N * 4 + (N + 3) * 20 + nres * 48 + (int64_t)sizeof(DevTables);
     // coarse grid upper bound: box/rl cells per axis at the densest tables
     b += 4 * N * 4 + N * 48 + 2 * 4 * 1024 * 1024;
+    b += N * 4;  // footprint marks
     return b;
 }
 
@@ -402,6 +405,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     HIPCHK(H.csorted.alloc(N));
     HIPCHK(H.spos.alloc(N));
     HIPCHK(H.sinfo.alloc(N));
+    HIPCHK(H.mark.alloc(N));
+    HIPCHK(hipMemsetAsync(H.mark.p, 0, sizeof(unsigned) * (size_t)N, ctx->stream));
     HIPCHK(H.cstart.alloc((size_t)Mq * Mq * Mq + 1));
     HIPCHK(H.crun.alloc((size_t)Mq * Mq * Mq));
     {
@@ -441,6 +446,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.csorted = H.csorted.p;
     D.spos = H.spos.p;
     D.sinfo = H.sinfo.p;
+    D.mark = H.mark.p;
     D.st = H.st.p;
     D.log = nullptr;
     H.alive = true;
@@ -550,7 +556,7 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
     long long ticks = 0;
     if (budget_ms > 0.0) ticks = (long long)(budget_ms * (double)ctx->wall_khz);
     HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit,
-                      ctx->stream, ticks, flags));
+                      ctx->stream, ticks, flags, ctx->waves));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     // the copies above read pageable memory synchronously before returning
@@ -560,6 +566,13 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
 }  // namespace pdmd
 
 extern "C" {
+
+int pdmd_set_waves(pdmd_ctx* ctx, int32_t waves_per_system) {
+    if (!ctx) return fail("pdmd_set_waves: null context");
+    if (waves_per_system < 1 || waves_per_system > 8) return fail("pdmd_set_waves: waves per system must be 1 to 8");
+    ctx->waves = waves_per_system;
+    return 0;
+}
 
 int pdmd_wait(pdmd_ctx* ctx) {
     if (!ctx) return fail("pdmd_wait: null context");

@@ -104,6 +104,7 @@ struct SysDev {
     int* csorted;  // [N]
     double4* spos; // [N] positions in coarse-cell order (list rebuild)
     int4* sinfo;   // [N] bead, topo word, fine cell, 0 in coarse-cell order
+    unsigned* mark;  // [N] per-round footprint marks of a multi-wavefront system (zero between rounds)
     SysState* st;
     LogRec* log;
 };

@@ -6,9 +6,11 @@
 
 namespace pdmd {
 
-// LDS bytes of one event-loop wavefront for a calendar of nblk 16-entry
-// blocks and a down-neighbour work list of maxd entries.
-size_t engine_lds_bytes(int nblk, int maxw, int leaf_cap);
+// LDS bytes of one system's workgroup of nw wavefronts for a calendar of
+// nblk 16-entry blocks, work lists of maxw entries and leaf_cap leaves in LDS.
+size_t engine_lds_bytes(int nblk, int maxw, int leaf_cap, int nw);
+// most wavefronts per system whose work areas fit in lds_limit
+int engine_max_waves(int max_n, int maxw, size_t lds_limit);
 // work-list capacity of the event kernels for a list capacity
 int engine_maxw(int cap);
 // Opt the event-loop kernels in to lds_limit bytes of LDS per workgroup.
@@ -17,10 +19,12 @@ hipError_t engine_prepare(size_t lds_limit);
 hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, hipStream_t s);
 // Event loop: every listed system runs until its SysState::target.
 // budget_ticks: wall_clock64 ticks after which each system stops between
-// events (0 = run to the event targets)
+// rounds (0 = run to the event targets); waves: wavefronts per system (1 to
+// 8, lowered to what fits in lds_limit), each round runs up to that many
+// conflict-free pair events side by side
 constexpr int kStopAtOutput = 1;  // engine_run flag: stop after the next output pseudo-event
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s, long long budget_ticks = 0, int flags = 0);
+                      size_t lds_limit, hipStream_t s, long long budget_ticks = 0, int flags = 0, int waves = 1);
 // Test hook: collision times of (i, j) pairs (1-based) of system id.
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s);

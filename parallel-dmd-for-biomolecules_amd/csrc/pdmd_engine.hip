@@ -31,6 +31,10 @@ constexpr int kNumbin = 2000;
 constexpr int kMaxBlist = 256;
 constexpr double kLt = 1e-10;
 constexpr double kSm = 5e-12;
+constexpr int kMaxWaves = 8;  // most wavefronts in one system's workgroup
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & (kW - 1)); }
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 // Phase timers (device clock cycles summed into SysState::phase[k]) when
 // built with -DPDMD_PHASE_TIMERS; no code otherwise.  Phases: 0 selection,
@@ -40,10 +44,12 @@ constexpr double kSm = 5e-12;
 // 10 ghost event before rescheduling, 11 reschedule rows and work list,
 // 12 reschedule down-set scan, 13 rebuild count, 14 sync count; list
 // rebuild steps: 16 cells and histogram, 17 scan, 18 scatter, 19 sorted
-// copies, 20 neighbour search.
+// copies, 20 neighbour search; multi-wavefront rounds: 21 rounds of two or
+// more pair events, 22 entries popped in them, 23 entries committed.
 enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
        kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
-       kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20 };
+       kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20,
+       kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23 };
 #ifdef PDMD_PHASE_TIMERS
 #define PH_BEGIN(v) long long v = clock64()
 #define PH_END(S, k, v)                                                                                  \
@@ -57,6 +63,10 @@ enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLe
         if (threadIdx.x == 0) atomicAdd((unsigned long long*)&(S).st->phase[k], 1ull);     \
     } while (0)
 #define PH_RESET(v) v = clock64()
+#define PH_ADDN(S, k, n)                                                                                  \
+    do {                                                                                                  \
+        if (threadIdx.x == 0) atomicAdd((unsigned long long*)&(S).st->phase[k], (unsigned long long)(n)); \
+    } while (0)
 #else
 #define PH_BEGIN(v) \
     do {            \
@@ -69,6 +79,9 @@ enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLe
     } while (0)
 #define PH_RESET(v) \
     do {            \
+    } while (0)
+#define PH_ADDN(S, k, n) \
+    do {                 \
     } while (0)
 #endif
 
@@ -371,13 +384,19 @@ __device__ inline Rec shfl_rec(const Rec& r, int src) {
     o.dyn = __shfl(r.dyn, src, kW);
     return o;
 }
-// Hand-off through LDS between lanes of the workgroup, which is one
-// wavefront: LDS operations of a wavefront complete in program order, so
-// only the compiler's ordering is needed (no wait for outstanding global
-// loads, unlike __syncthreads).  Global-memory hand-offs keep
-// __threadfence_block + __syncthreads.
+// Hand-off through LDS between the lanes of one wavefront: LDS operations
+// of a wavefront complete in program order, so only the compiler's ordering
+// is needed (no wait for outstanding global loads).
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+// Hand-off of global-memory and LDS writes between the lanes of one
+// wavefront (the stores complete before any lane reads past this point).
+// Event-level code synchronises only its own wavefront: the other
+// wavefronts of a system's workgroup meet it at the round barriers.
+__device__ inline void wave_sync_mem() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -557,11 +576,25 @@ __device__ inline unsigned long long t_order(double t) {
 }
 
 
-// LDS views
+// One round of a multi-wavefront system: the popped calendar entries (in
+// key order, one per wavefront) and what each wavefront reports back.
+struct Batch {
+    Key key[kMaxWaves];   // popped entries
+    Key knew[kMaxWaves];  // least key among the wavefront's new calendar entries
+    int ty[kMaxWaves];    // resolved type of each event
+    int nbl[kMaxWaves];   // blocks to rescan, in each wavefront's list
+    int st[kMaxWaves];    // status after each event
+    int K;                // entries popped this round (0: stop)
+    int cut;              // first wavefront whose event is not committed
+};
+
+// LDS views, one set per wavefront: the calendar tree, the master scalars
+// and the round record are shared, the work lists are the wavefront's own
 struct Lds {
     int o_blk, o_sup, o_wm, o_wx, o_blist, o_pair, o_sc, o_leaf, o_wd, o_wt, o_wy, o_slot, o_spec, o_down;
+    int o_msc, o_batch;
     Leaf* gleaf;  // calendar leaves in HBM when they do not fit in LDS
-    int nblk, nsup, maxw;
+    int nblk, nsup, maxw, nw;
     bool leaf_lds;
     __device__ Key* blk() const { return reinterpret_cast<Key*>(smem + o_blk); }
     __device__ Key* sup() const { return reinterpret_cast<Key*>(smem + o_sup); }
@@ -576,11 +609,14 @@ struct Lds {
     __device__ DownInfo* down() const { return reinterpret_cast<DownInfo*>(smem + o_down); }
     __device__ Pair* pair() const { return reinterpret_cast<Pair*>(smem + o_pair); }
     __device__ Sc* sc() const { return reinterpret_cast<Sc*>(smem + o_sc); }
+    __device__ Sc* msc() const { return reinterpret_cast<Sc*>(smem + o_msc); }
+    __device__ Batch* batch() const { return reinterpret_cast<Batch*>(smem + o_batch); }
     __device__ Leaf* lleaf() const { return reinterpret_cast<Leaf*>(smem + o_leaf); }
 };
-// the Lds views themselves, after the SysDev copy
+// the Lds views themselves (one per wavefront), after the SysDev copy
 constexpr int kLdOff = kSdOff + (int)((sizeof(SysDev) + 15) / 16 * 16);
-__device__ inline const Lds& ldsv() { return *reinterpret_cast<const Lds*>(smem + kLdOff); }
+constexpr int kLdStride = (int)((sizeof(Lds) + 15) / 16 * 16);
+__device__ inline const Lds& ldsv() { return *reinterpret_cast<const Lds*>(smem + kLdOff + wave_id() * kLdStride); }
 __device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : gld(L.gleaf + k); }
 __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
     if (L.leaf_lds)
@@ -590,26 +626,31 @@ __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
 }
 
 // block minima of the listed blocks (4 blocks per pass, one row of 16 lanes
-// each), then the superblock minima over the touched superblocks
-__device__ void rescan_block_list(const Lds&, int N3, int nb) {
+// each); a block listed twice gets the same value twice
+__device__ void rescan_blocks(const Lds&, int N3, const int* list, int nb) {
     const Lds& L = ldsv();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     for (int base = 0; base < nb; base += 4) {
         const int q = base + (lane >> 4), k = lane & 15;
-        const int b = q < nb ? L.blist()[q] : -1;
+        const int b = q < nb ? list[q] : -1;
         Key e = empty_key();
         const int idx = b * kBlk + k;
         if (b >= 0 && idx < N3) e = leaf_key(leaf_get(L, idx), idx);
         key_min_rows(e);
         if (k == 15 && b >= 0) L.blk()[b] = e;
     }
-    wave_sync();
-    // the touched superblocks (at most 64), each reduced once
+}
+// superblock minima over the superblocks of the listed blocks (at most 64),
+// each reduced once
+__device__ void rescan_sups(const Lds&, const int* list, int nb) {
+    const Lds& L = ldsv();
+
+    const int lane = lane_id();
     unsigned long long touched = 0;
     for (int base = 0; base < nb; base += kW) {
         const int q = base + lane;
-        const int sq = q < nb ? L.blist()[q] / kSup : 0;
+        const int sq = q < nb ? list[q] / kSup : 0;
         const int nr = min(nb - base, kW);
         for (int r = 0; r < nr; ++r) touched |= 1ull << __builtin_amdgcn_readlane(sq, r);
     }
@@ -620,12 +661,20 @@ __device__ void rescan_block_list(const Lds&, int N3, int nb) {
         const Key e = key_min_wave(b < L.nblk ? L.blk()[b] : empty_key());
         if (lane == 0) L.sup()[s] = e;
     }
+}
+// both levels for the wavefront's block list
+__device__ void rescan_block_list(const Lds&, int N3, int nb) {
+    const Lds& L = ldsv();
+
+    rescan_blocks(L, N3, L.blist(), nb);
+    wave_sync();
+    rescan_sups(L, L.blist(), nb);
     wave_sync();
 }
 __device__ void rescan_all(const Lds&, int N3) {
     const Lds& L = ldsv();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     for (int base = 0; base < L.nblk; base += 4) {
         const int b = base + (lane >> 4), k = lane & 15;
         Key e = empty_key();
@@ -634,13 +683,92 @@ __device__ void rescan_all(const Lds&, int N3) {
         key_min_rows(e);
         if (k == 15 && b < L.nblk) L.blk()[b] = e;
     }
-    __syncthreads();
+    wave_sync_mem();
     for (int s = 0; s < L.nsup; ++s) {
         const int b = s * kSup + lane;
         const Key e = key_min_wave(b < L.nblk ? L.blk()[b] : empty_key());
         if (lane == 0) L.sup()[s] = e;
     }
-    __syncthreads();
+    wave_sync_mem();
+}
+
+// ------------------------------------------------------------------------
+// multi-wavefront rounds: footprint marks and the accepted prefix
+// ------------------------------------------------------------------------
+// shoulder beads of an H-bond pair (repuls_add.f:16-25): N bead i, C bead j
+__device__ inline void shoulder_idx(int i, int Li, int j, int Lj, int& ncim1, int& ncai, int& ncaj, int& nnjp1) {
+    ncim1 = i + Li - 1;
+    ncai = i - Li;
+    ncaj = j - 2 * Lj;
+    nnjp1 = j - Lj + 1;
+}
+
+// Lane's bead of the shoulder set an event on (i, j) may read or rewrite
+// (main.F90:1487-1634 via repuls_check/_3, repuls_add/_del_a/_del_b and the
+// H-bond partner's identity): the H-bond partners of i and j (lanes 0, 1)
+// and the shoulder beads of (i,j), (j,i), (i,hbp(i)), (hbp(i),i),
+// (j,hbp(j)), (hbp(j),j) (lanes 2..25); -1 where there is none
+__device__ inline int shoulder_bead(const SysDev&, const Pair& P, int lane) {
+    const SysDev& S = sdev();
+    const int hi = P.xi.w, hj = P.j >= 0 ? P.xj.w : -1;
+    if (lane == 0) return hi;
+    if (lane == 1) return hj;
+    if (lane >= 26 || P.j < 0) return -1;
+    const int g = (lane - 2) >> 2, q = (lane - 2) & 3;
+    const int x = g < 2 ? P.i : (g < 4 ? P.i : P.j);
+    const int y = g < 2 ? P.j : (g < 4 ? hi : hj);
+    if (y < 0) return -1;
+    const int tx = x == P.i ? P.ri.topo : P.rj.topo;
+    const int ty = y == P.i ? P.ri.topo : (y == P.j ? P.rj.topo : gl1(&S.rec[y].topo));
+    const bool swap = (g & 1) != 0;
+    const int n = swap ? y : x, cb = swap ? x : y;
+    const int Ln = topo_len(swap ? ty : tx), Lc = topo_len(swap ? tx : ty);
+    int a[4];
+    shoulder_idx(n, Ln, cb, Lc, a[0], a[1], a[2], a[3]);
+    const int v = q == 0 ? a[0] : (q == 1 ? a[1] : (q == 2 ? a[2] : a[3]));
+    return (v >= 0 && v < S.N) ? v : -1;
+}
+// the marks live in HBM and are read and written at the device's L2
+__device__ inline void mark_or(unsigned* p, unsigned bits) {
+    __hip_atomic_fetch_or((GLB unsigned*)p, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned mark_get(const unsigned* p) {
+    return __hip_atomic_load((GLB unsigned*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void mark_clear(unsigned* p) {
+    __hip_atomic_store((GLB unsigned*)p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A wavefront's place in a multi-wavefront round: -1 alone; 0 the round's
+// first event (always committed, writes as it goes); k >= 1 a speculative
+// event whose writes wait for the round's checks.  In a round every
+// wavefront passes the three round barriers once (footprints marked,
+// footprints checked, new calendar keys known), whichever way it leaves.
+__device__ inline void bar_to(int& nb, int n) {
+    while (nb < n) {
+        __syncthreads();
+        ++nb;
+    }
+}
+__device__ inline int spec_abort(const Lds& L, int bk, int& nb) {
+    if (lane_id() == 0) atomicMin(&L.batch()->cut, bk);
+    bar_to(nb, 3);
+    return -1;
+}
+
+// Events of a round committed: the prefix before the first cut-off event,
+// before the first event after an error, and before the first event whose
+// key is not below every new calendar key of the events before it (the
+// serial order would have run one of those first).
+__device__ inline int batch_accepted(const Batch& B) {
+    int A = min(B.K, B.cut);
+    Key lo = B.knew[0];
+    if (B.st[0] != 0) A = min(A, 1);
+    for (int k = 1; k < A; ++k) {
+        if (!key_less(B.key[k], lo)) return k;
+        if (B.st[k] != 0) return k + 1;
+        if (key_less(B.knew[k], lo)) lo = B.knew[k];
+    }
+    return A;
 }
 
 // ------------------------------------------------------------------------
@@ -664,7 +792,7 @@ __device__ void full_one(const SysDev&, const Lds&, int l, const Rec& rl, double
 
     const DevTables& T = tabs();
     const Pair& P = *L.pair();
-    const int lane = threadIdx.x, cap = S.cap;
+    const int lane = lane_id(), cap = S.cap;
     const int nu = gl1(S.nup + (l));
     const int nx = (xl.x > l ? 1 : 0) + (xl.y > l ? 1 : 0) + (xl.z > l ? 1 : 0);
     const int tot = nu + nx;
@@ -755,7 +883,7 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
     Sc& c = *L.sc();
     const Pair& P = *L.pair();
 
-    const int lane = threadIdx.x, cap = S.cap, N3 = S.N + 3;
+    const int lane = lane_id(), cap = S.cap, N3 = S.N + 3;
     const int i = P.i, j = P.j;
     const double tf = c.tfalse;
     const double T0 = c.interval_max + kLt - tf;
@@ -817,7 +945,7 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
     }
     const int nD = w - ncand;
     if (w > L.maxw) status = kErrListCapacity;
-    __syncthreads();
+    wave_sync_mem();
     PH_END(S, kPhLoads, tph);
     // ---- candidate predictions of the full recomputes of i and j
     double bt0 = 1e300, bt1 = 1e300;
@@ -1008,8 +1136,10 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
-    if (!L.leaf_lds) __threadfence_block();
-    __syncthreads();
+    if (!L.leaf_lds)
+        wave_sync_mem();
+    else
+        wave_sync();
     PH_END(S, kPhLeaf, tph);
     rescan_block_list(L, N3, nbl);
     PH_END(S, kPhRescan, tph);
@@ -1027,7 +1157,7 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
 // the generic path.  extra_blk: a further calendar block to rescan (the
 // ghost entry), or -1
 __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, const Pair&, int extra_blk,
-                           bool xpulse_del) {
+                           bool xpulse_del, int bk) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
@@ -1036,12 +1166,13 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     Sc& c = *L.sc();
     const Pair& P = *L.pair();
 
-    const int lane = threadIdx.x, cap = S.cap, N3 = S.N + 3;
+    const int lane = lane_id(), cap = S.cap, N3 = S.N + 3;
     const int i = P.i, j = P.j;
     const double tf = c.tfalse;
     const double T0 = c.interval_max + kLt - tf;
     const int dbase = L.maxw - (2 * cap + 8);  // down-set region of the work list
-    int status = 0;
+    const bool batch = bk >= 0;
+    int status = 0, nb = 0;
     PH_BEGIN(tph);
     // ---- round 1: list rows of i and j, whole rows with their lengths
     const int jr = j >= 0 ? j : i;
@@ -1102,8 +1233,19 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     }
     const int nD = d;
     if (nD > kW) {
-        __syncthreads();
+        // the generic form writes as it goes: it runs only as a round's
+        // first event, and no other event of the round is committed
+        if (bk >= 1) {
+            spec_abort(L, bk, nb);
+            return;
+        }
+        if (batch && lane == 0) atomicMin(&L.batch()->cut, 1);
+        wave_sync_mem();
         reschedule_generic(S, T, L, c, P, extra_blk, xpulse_del);
+        if (batch) {
+            if (lane == 0) L.batch()->nbl[bk] = 0;
+            bar_to(nb, 3);
+        }
         return;
     }
     if (lane < 2 + kMaxSpec) {
@@ -1116,9 +1258,10 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     //      bead's op counts, and whether its full recompute is batched
     int l = -1, cI = 0, cJ = 0;
     bool first = false;
+    const int dent = lane < nD ? L.wm()[dbase + lane] : -1;
     {
         // every lane compares its bead with each entry, read from lanes
-        const int lv = lane < nD ? L.wm()[dbase + lane] : -1;
+        const int lv = dent;
         const int tv = lane < nD ? L.wx()[dbase + lane] : 0;
         l = lv;
         first = lane < nD;
@@ -1227,6 +1370,53 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                     if (x.z > ls) { L.wm()[k] = x.z | kNoStat; L.wx()[k] = 2 + s2; ++k; }
                 }
             }
+        }
+    }
+    // ---- a round's footprint check.  Marks per bead: bit k = wavefront k
+    //      rewrites the bead's state, bit 8+k = it rewrites the bead's
+    //      calendar entry.  Event k commits only if no earlier event of the
+    //      round rewrites a state it reads or an entry it rewrites; a later
+    //      event reads the state before this one's writes (they wait for the
+    //      round's last barrier), as in the serial order.
+    if (batch) {
+        Batch& B = *L.batch();
+        const unsigned sbit = 1u << bk, lbit = 1u << (8 + bk);
+        const int sp = shoulder_bead(S, P, lane);
+        if (dent >= 0) mark_or(S.mark + dent, lbit);
+        if (lane == 0) mark_or(S.mark + i, sbit | lbit);
+        if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit);
+        if (bk == 0 && sp >= 0) mark_or(S.mark + sp, sbit);
+        bar_to(nb, 1);
+        if (bk >= 1) {
+            const unsigned lowS = sbit - 1u, lowL = lowS << 8;
+            bool bad = false;
+            for (int base = 0; base < npairs; base += kW) {
+                const int q = base + lane;
+                if (q < npairs) {
+                    const int m = (base == 0 && q < ncand ? cent : L.wm()[q]) & 0xFFFFFF;
+                    if (mark_get(S.mark + m) & lowS) bad = true;
+                }
+            }
+            if (dent >= 0 && (mark_get(S.mark + dent) & (lowS | lowL))) bad = true;
+            if (lane == 0 && (mark_get(S.mark + i) & (lowS | lowL))) bad = true;
+            if (lane == 1 && j >= 0 && (mark_get(S.mark + j) & (lowS | lowL))) bad = true;
+            if (sp >= 0 && (mark_get(S.mark + sp) & lowS)) bad = true;
+            // a down bead whose full recompute is not in the batch would
+            // read rows that were not checked
+            const bool need_full = first && !is_i &&
+                                   ((lf0.nptnr == i && cI > 0) || (lf0.nptnr == j && cJ > 0) || cI >= 2 || cJ >= 2);
+            if (need_full && !(spec && L.spec()[myslot].off >= 0)) bad = true;
+            if (__ballot(bad) && lane == 0) atomicMin(&B.cut, bk);
+        }
+        bar_to(nb, 2);
+        // every wavefront has checked: clear this one's marks
+        if (dent >= 0) mark_clear(S.mark + dent);
+        if (lane == 0) mark_clear(S.mark + i);
+        if (lane == 1 && j >= 0) mark_clear(S.mark + j);
+        if (bk == 0 && sp >= 0) mark_clear(S.mark + sp);
+        if (bk >= B.cut) {
+            bar_to(nb, 3);
+            return;
         }
     }
     // ---- the single-pair predictions (l, i) and (l, j) of each down bead,
@@ -1438,13 +1628,50 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         ci = __shfl(st_c, src, kW);
     }
     const bool wr = modified && !is_i;
+    Leaf lf, la, lb;
     if (wr) {
-        Leaf lf;
         lf.tim = st_t;
         lf.nptnr = st_p;
         lf.meta = make_meta(calc_bin(c, st_t, false, status), st_c);
-        leaf_set(L, l, lf);
     }
+    if (lane == 0) {
+        la.tim = ti;
+        la.nptnr = pi;
+        la.meta = make_meta(calc_bin(c, ti, false, status), ci);
+        if (j >= 0) {
+            lb.tim = tj;
+            lb.nptnr = pj;
+            lb.meta = make_meta(calc_bin(c, tj, false, status), cj);
+        }
+    }
+    status = wave_status(status);
+    if (status && !c.status) c.status = status;
+    if (batch) {
+        // report the least new calendar key and the status, then commit
+        // only within the round's accepted prefix
+        Batch& B = *L.batch();
+        Key kn = wr ? leaf_key(lf, l) : empty_key();
+        if (lane == 0) {
+            const Key ka = leaf_key(la, i);
+            if (key_less(ka, kn)) kn = ka;
+            if (j >= 0) {
+                const Key kb = leaf_key(lb, j);
+                if (key_less(kb, kn)) kn = kb;
+            }
+        }
+        kn = key_min_wave(kn);
+        if (lane == 0) {
+            B.knew[bk] = kn;
+            B.st[bk] = c.status;
+        }
+        bar_to(nb, 3);
+        if (bk >= batch_accepted(B)) return;
+        if (bk >= 1 && lane == 0) {
+            gst(S.rec + (i), P.ri);
+            gst(S.rec + (j), P.rj);
+        }
+    }
+    if (wr) leaf_set(L, l, lf);
     if (lane == 0) {
         L.blist()[0] = i / kBlk;
         L.blist()[1] = j >= 0 ? j / kBlk : i / kBlk;
@@ -1456,18 +1683,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const int nbl = 3 + __popcll(mm);
     PH_END(S, kPhDown, tph);
     if (lane == 0) {
-        Leaf a;
-        a.tim = ti;
-        a.nptnr = pi;
-        a.meta = make_meta(calc_bin(c, ti, false, status), ci);
-        leaf_set(L, i, a);
-        if (j >= 0) {
-            Leaf b;
-            b.tim = tj;
-            b.nptnr = pj;
-            b.meta = make_meta(calc_bin(c, tj, false, status), cj);
-            leaf_set(L, j, b);
-        }
+        leaf_set(L, i, la);
+        if (j >= 0) leaf_set(L, j, lb);
     }
     // repuls_del_b.f once the rescheduling is done
     if (xpulse_del && lane == 0) {
@@ -1489,11 +1706,13 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         gs1(&S.extra[ncai].z, -1);
         gs1(&S.extra[ncim1].z, -1);
     }
-    status = wave_status(status);
-    if (status && !c.status) c.status = status;
+    if (batch) {
+        // the round's block rescans run after every committed event's writes
+        if (lane == 0) L.batch()->nbl[bk] = nbl;
+        return;
+    }
     if (!L.leaf_lds || xpulse_del) {
-        __threadfence_block();
-        __syncthreads();
+        wave_sync_mem();
     } else
         wave_sync();
     PH_END(S, kPhLeaf, tph);
@@ -1649,12 +1868,6 @@ __device__ inline double dist_rec(const Rec& a, const Rec& b, double tf) {
     geo(a, b, tf, g);
     return sqrt(g.rx * g.rx + g.ry * g.ry + g.rz * g.rz);
 }
-__device__ inline void shoulder_idx(int i, int Li, int j, int Lj, int& ncim1, int& ncai, int& ncaj, int& nnjp1) {
-    ncim1 = i + Li - 1;
-    ncai = i - Li;
-    ncaj = j - 2 * Lj;
-    nnjp1 = j - Lj + 1;
-}
 // repuls_check.f:16-81 (n = the N bead, cb = the C bead)
 __device__ double repuls_check(const SysDev&, const DevTables& T, const Pair& P, int n, int cb, double tf) {
     const SysDev& S = sdev();
@@ -1731,7 +1944,7 @@ __device__ inline bool terminal_t(int t) {
 
 // One pair event: outcome resolution (main.F90:1487-1634), dynamics, H-bond
 // and shoulder bookkeeping (main.F90:641-921), then partial_events.
-__device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int i, int j, int ct) {
+__device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int i, int j, int ct, int bk) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
@@ -1739,7 +1952,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const double tf = c.tfalse;
     PH_BEGIN(tpe);
     Pair P;
@@ -1763,6 +1976,9 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
                 const double rating = idi < idj ? repuls_check(S, T, P, i, j, tf) : repuls_check(S, T, P, j, i, tf);
                 ct = rating <= 10.0 ? 4 : 14;
             } else {
+                // a random draw: its counter value is known only in order
+                int nb = 0;
+                if (bk >= 1) return spec_abort(L, bk, nb);
                 const double ran_non = rng_u01(S.seed, 1, c.rc1++);
                 ct = ran_non <= 0.2 ? 4 : 9;
             }
@@ -1801,6 +2017,8 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
             P.ri.dyn = make_dyn(idi + 4, j);
             P.rj.dyn = make_dyn(idj + 4, i);
             if (ok) {
+                int nb = 0;
+                if (bk >= 1) return spec_abort(L, bk, nb);
                 if (lane == 0) {
                     if (idi < idj) repuls_add(S, i, Li, j, Lj, status);
                     else repuls_add(S, j, Lj, i, Li, status);
@@ -1811,6 +2029,8 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     } else if (ct == 21) {
         if (idi <= 8) {
             if (ok) {
+                int nb = 0;
+                if (bk >= 1) return spec_abort(L, bk, nb);
                 if (lane == 0) {
                     if (idi < idj) repuls_del_a(S, i, Li, j, Lj, status);
                     else repuls_del_a(S, j, Lj, i, Li, status);
@@ -1846,6 +2066,8 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
             if (hbp == i) P.ri.dyn = hdyn;
             else if (hbp == j) P.rj.dyn = hdyn;
             else {
+                int nb = 0;
+                if (bk >= 1) return spec_abort(L, bk, nb);
                 if (lane == 0) gs1(&S.rec[hbp].dyn, hdyn);
                 fence = true;
             }
@@ -1856,6 +2078,10 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
         geo(P.ri, P.rj, tf, g);
         const double bd = evcode >= 40 ? kSm * sqrt(T.shlddia_sq[idi][idj]) : kSm * sqrt(T.welldia_sq[idi][idj]);
         nudge(P.ri, P.rj, bd, g, g.b < 0.0 ? -1 : +1);
+        if ((ct == 14 || ct == 16) && ok && bk >= 1) {
+            int nb = 0;
+            return spec_abort(L, bk, nb);
+        }
         if (ct == 14 && ok) {
             if (lane == 0) {
                 if (idi < idj) repuls_add(S, i, Li, j, Lj, status);
@@ -1871,14 +2097,14 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
             fence = true;
         }
     }
-    if (lane == 0) {
+    if (lane == 0 && bk < 1) {  // a speculative event writes its pair once committed
         gst(S.rec + (i), P.ri);
         gst(S.rec + (j), P.rj);
     }
+    if (lane == 0 && bk >= 0) L.batch()->ty[bk] = ct;
     if (fence) {
         // shoulder tables changed: make them visible before the predictions
-        __threadfence_block();
-        __syncthreads();
+        wave_sync_mem();
         P.xi = gld(S.extra + (i));
         P.xj = gld(S.extra + (j));
     }
@@ -1887,7 +2113,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     if (lane == 0) *L.pair() = P;
     wave_sync();
     PH_END(S, kPhPair, tpe);
-    reschedule(S, T, L, c, *L.pair(), -1, xpulse);
+    reschedule(S, T, L, c, *L.pair(), -1, xpulse, bk);
     return ct;
 }
 
@@ -1900,7 +2126,7 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int N = S.N;
     PH_BEGIN(tg0);
     int i = 0, guard = 0;
@@ -1963,7 +2189,7 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
     if (lane == 0) *L.pair() = P;
     wave_sync();
     PH_END(S, kPhGhost, tg0);
-    reschedule(S, T, L, c, *L.pair(), N / kBlk, false);
+    reschedule(S, T, L, c, *L.pair(), N / kBlk, false, -1);
     return i;
 }
 
@@ -1985,7 +2211,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     const SysDev& S = sdev();
 
     const DevTables& T = tabs();
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int N = S.N, M = S.M, Mq = S.Mq, NC = S.ncoarse, cap = S.cap;
     PH_BEGIN(trl);
 #pragma unroll 4
@@ -2005,11 +2231,9 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         gs1(S.cstart + (q), 0);
         gs1(S.crun + (q), 0);
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
     for (int k = lane; k < N; k += kW) atomicAdd(&S.cstart[gl1(S.ckey + k)], 1);
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
     PH_END(S, kPhLCells, trl);
     // exclusive scan of the cell counts
     int carry = 0;
@@ -2027,8 +2251,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         carry += __builtin_amdgcn_readlane(incl, kW - 1);
     }
     if (lane == 0) gs1(S.cstart + (NC), carry);
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
     PH_END(S, kPhLScan, trl);
     // stable scatter in bead order: rank among the lower lanes of the same cell
     for (int base = 0; base < N; base += kW) {
@@ -2044,14 +2267,12 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
             }
         }
         const int basepos = k < N ? gl1(S.cstart + (key)) + gl1(S.crun + (key)) : 0;
-        __threadfence_block();
-        __syncthreads();
+        wave_sync_mem();
         if (k < N) {
             gs1(S.csorted + (basepos + rank), k);
             if (rank == cnt - 1) gs1(S.crun + (key), gl1(S.crun + (key)) + cnt);
         }
-        __threadfence_block();
-        __syncthreads();
+        wave_sync_mem();
     }
     PH_END(S, kPhLScatter, trl);
     // positions, topology and fine cells in sorted order
@@ -2062,8 +2283,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         gst(S.spos + (p2), make_double4(b.x, b.y, b.z, 0.0));
         gst(S.sinfo + (p2), make_int4(jb, b.topo, gl1(S.cfine + jb), 0));
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
     PH_END(S, kPhLCopy, trl);
     int mx = 0, status = 0;
     const int noff = Mq > 2 ? 3 : Mq;  // distinct coarse offsets {0, +1, -1}
@@ -2182,8 +2402,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     if (status && !c.status) c.status = status;
     mx = wave_imax(mx);
     if (lane == 0 && mx > S.st->max_nb_seen) S.st->max_nb_seen = mx;
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
 }
 
 // events.f:11-127 with tfalse = 0: every bead's next event, then the calendar
@@ -2195,7 +2414,7 @@ __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int N = S.N, cap = S.cap;
     int status = 0;
     c.nbin = 1;
@@ -2252,8 +2471,10 @@ __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
-    if (!L.leaf_lds) __threadfence_block();
-    __syncthreads();
+    if (!L.leaf_lds)
+        wave_sync_mem();
+    else
+        wave_sync();
     rescan_all(L, N + 3);
 }
 
@@ -2265,7 +2486,7 @@ __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     for (int k = lane; k < S.N; k += kW) {
         Rec b = gld(S.rec + (k));
         b.x = b.x - round(b.x);
@@ -2274,8 +2495,7 @@ __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
         gst(S.rec + (k), b);
         gst(S.old_r + (k), make_double4(b.x, b.y, b.z, 0.0));
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
     PH_BEGIN(trb);
     rebuild_lists(S, L, c);
     PH_END(S, kPhLists, trb);
@@ -2293,7 +2513,7 @@ __device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const DevTables& T = tabs();
     Sc& c = *L.sc();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int N = S.N;
     const double tf = c.tfalse;
     PH_BEGIN(tsy);
@@ -2332,8 +2552,7 @@ __device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
             c.interval = c.t_fact / sqrt(S.setemp);
         }
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync_mem();
     if (update || c.interval > c.interval_max) {
         if (!update) {
             c.nforced++;
@@ -2352,7 +2571,7 @@ __device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
         for (int b = lane; b < L.nblk; b += kW)
             if (L.blk()[b].bc != kEmptyBc) L.blk()[b].t = L.blk()[b].t - tf;
         if (lane < L.nsup && L.sup()[lane].bc != kEmptyBc) L.sup()[lane].t = L.sup()[lane].t - tf;
-        __syncthreads();
+        wave_sync_mem();
     }
     int status = 0;
     const double t2 = c.interval * 0.999;
@@ -2365,8 +2584,10 @@ __device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
         L.blist()[0] = (N + 1) / kBlk;
     }
     if (status && !c.status) c.status = status;
-    if (!L.leaf_lds) __threadfence_block();
-    __syncthreads();
+    if (!L.leaf_lds)
+        wave_sync_mem();
+    else
+        wave_sync();
     rescan_block_list(L, N + 3, 1);
     PH_END(S, kPhSync, tsy);
 }
@@ -2378,7 +2599,7 @@ __device__ void output_event(const SysDev&, const Lds&, Sc&) {
 
     Sc& c = *L.sc();
 
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int N = S.N;
     int status = 0;
     const double t3 = 3.3 / (sqrt(S.setemp)) + 5 + c.tfalse;
@@ -2391,12 +2612,18 @@ __device__ void output_event(const SysDev&, const Lds&, Sc&) {
         L.blist()[0] = (N + 2) / kBlk;
     }
     if (status && !c.status) c.status = status;
-    if (!L.leaf_lds) __threadfence_block();
-    __syncthreads();
+    if (!L.leaf_lds)
+        wave_sync_mem();
+    else
+        wave_sync();
     rescan_block_list(L, N + 3, 1);
 }
 
-__device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap) {
+// LDS plan: tables, SysDev, the views, the shared calendar tree, master
+// scalars and round record, then each wavefront's work area (wavefront 0's
+// is the larger: the list rebuild and the generic rescheduling use it as
+// scratch; its scalars are the master copy), then the leaves
+__device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap, int nw, int wave) {
     const SysDev& S = sdev();
 
     Lds L;
@@ -2404,28 +2631,46 @@ __device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap) {
     L.nblk = (N3 + kBlk - 1) / kBlk;
     L.nsup = (L.nblk + kSup - 1) / kSup;
     L.maxw = maxw;
+    L.nw = nw;
     int p = 0;
     auto take = [&](int bytes) {
         const int o = p;
         p = (p + bytes + 15) / 16 * 16;
         return o;
     };
-    take((int)sizeof(DevTables));  // the tables, at offset 0
-    take((int)sizeof(SysDev));     // the system's SysDev, at kSdOff
-    take((int)sizeof(Lds));        // these views, at kLdOff
+    take((int)sizeof(DevTables));        // the tables, at offset 0
+    take((int)sizeof(SysDev));           // the system's SysDev, at kSdOff
+    take(kLdStride * kMaxWaves);         // these views, at kLdOff
     L.o_blk = take((int)sizeof(Key) * nblk_max);
     L.o_sup = take((int)sizeof(Key) * kSup);
-    L.o_wm = take((int)sizeof(int) * maxw);
-    L.o_wx = take((int)sizeof(int) * maxw);
-    L.o_blist = take((int)sizeof(int) * kMaxBlist);
-    L.o_wd = take((int)sizeof(int) * maxw);
-    L.o_wy = take((int)sizeof(int) * maxw);
-    L.o_wt = take((int)sizeof(double) * maxw);
-    L.o_slot = take((int)sizeof(MinSlot) * (2 + kMaxSpec));
-    L.o_spec = take((int)sizeof(SpecInfo) * kMaxSpec);
-    L.o_down = take((int)sizeof(DownInfo) * kW);
-    L.o_pair = take((int)sizeof(Pair));
-    L.o_sc = take((int)sizeof(Sc));
+    L.o_msc = take((int)sizeof(Sc));
+    L.o_batch = take((int)sizeof(Batch));
+    for (int w = 0; w <= wave; ++w) {
+        L.o_wm = take((int)sizeof(int) * maxw);
+        L.o_wx = take((int)sizeof(int) * maxw);
+        L.o_blist = take((int)sizeof(int) * kMaxBlist);
+        L.o_wd = take((int)sizeof(int) * (w == 0 ? maxw : 16));
+        L.o_wy = take((int)sizeof(int) * maxw);
+        L.o_wt = take((int)sizeof(double) * maxw);
+        L.o_slot = take((int)sizeof(MinSlot) * (2 + kMaxSpec));
+        L.o_spec = take((int)sizeof(SpecInfo) * kMaxSpec);
+        L.o_down = take((int)sizeof(DownInfo) * kW);
+        L.o_pair = take((int)sizeof(Pair));
+        L.o_sc = w == 0 ? L.o_msc : take((int)sizeof(Sc));
+    }
+    for (int w = wave + 1; w < nw; ++w) {
+        take((int)sizeof(int) * maxw);
+        take((int)sizeof(int) * maxw);
+        take((int)sizeof(int) * kMaxBlist);
+        take((int)sizeof(int) * 16);
+        take((int)sizeof(int) * maxw);
+        take((int)sizeof(double) * maxw);
+        take((int)sizeof(MinSlot) * (2 + kMaxSpec));
+        take((int)sizeof(SpecInfo) * kMaxSpec);
+        take((int)sizeof(DownInfo) * kW);
+        take((int)sizeof(Pair));
+        take((int)sizeof(Sc));
+    }
     L.o_leaf = p;
     L.leaf_lds = N3 <= leaf_cap;
     L.gleaf = S.leaf;
@@ -2478,7 +2723,7 @@ __device__ void copy_tables(const DevTables* src, DevTables* dst) {
     const double* s = (const double*)src;
     double* d = (double*)dst;
     const int n = (int)(sizeof(DevTables) / sizeof(double));
-    for (int k = threadIdx.x; k < n; k += kW) d[k] = s[k];
+    for (int k = (int)threadIdx.x; k < n; k += (int)blockDim.x) d[k] = s[k];
 }
 
 }  // namespace
@@ -2492,7 +2737,7 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
     const SysDev& S = sdev();
-    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, nblk_max, maxw, 0);
+    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, nblk_max, maxw, 0, 1, 0);
     __syncthreads();
     const Lds& L = ldsv();
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
@@ -2527,83 +2772,221 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
     if (threadIdx.x == 0) store_sc(S.st, c);
 }
 
-// The event loop: each wavefront commits events of its system until the
-// system's coll counter has advanced by inc[k].
+// Pop up to kmax calendar entries in key order (wavefront 0): pair events
+// only after the first, since a pseudo-event runs alone.  Each popped entry
+// but the last is taken out of the min tree until the round's rescans.
+// Every speculative wavefront k gets a copy of the master scalars as they
+// stand when its event comes up in the serial order; the master takes the
+// first event.  Returns the number popped (0: stop, or an empty calendar).
+__device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
+    const Lds& L = ldsv();
+
+    Batch& B = *L.batch();
+    const int lane = lane_id();
+    int K = 0;
+    while (true) {
+        const Key k = key_min_wave(lane < L.nsup ? L.sup()[lane] : empty_key());
+        if (k.bc == kEmptyBc) {
+            if (K == 0) c.status = kErrBin;
+            break;
+        }
+        const int e = key_idx(k);
+        if (K > 0 && e >= N) break;
+        if (lane == 0) B.key[K] = k;
+        ++K;
+        if (e >= N || K >= kmax) break;
+        wave_sync();
+        const int b = e / kBlk, sp = b / kSup;
+        Key x = empty_key();
+        const int idx = b * kBlk + (lane & 15);
+        if (lane < kBlk && idx < N3) {
+            bool popped = false;
+            for (int m = 0; m < K; ++m) popped |= key_idx(B.key[m]) == idx;
+            if (!popped) x = leaf_key(leaf_get(L, idx), idx);
+        }
+        key_min_rows(x);
+        if (lane == 15) L.blk()[b] = x;
+        wave_sync();
+        const int bb = sp * kSup + lane;
+        const Key y = key_min_wave(bb < L.nblk ? L.blk()[bb] : empty_key());
+        if (lane == 0) L.sup()[sp] = y;
+        wave_sync();
+    }
+    wave_sync();
+    if (K == 0) return 0;
+    for (int k = 1; k < K; ++k) {
+        const Lds& Lk = *reinterpret_cast<const Lds*>(smem + kLdOff + k * kLdStride);
+        Sc* ck = Lk.sc();
+        if (lane < (int)(sizeof(Sc) / 8))
+            reinterpret_cast<long long*>(ck)[lane] = reinterpret_cast<const long long*>(&c)[lane];
+        wave_sync();
+        if (lane == 0) {
+            ck->old_tfalse = B.key[k - 1].t;
+            ck->tfalse = B.key[k].t;
+            ck->nbin = key_bin(B.key[k]);
+            ck->coll = c.coll + 1 + k;
+            ck->status = 0;
+        }
+    }
+    wave_sync();
+    const Key k0 = B.key[0];
+    c.old_tfalse = c.tfalse;
+    c.nbin = key_bin(k0);
+    c.tfalse = k0.t;
+    c.coll++;
+    return K;
+}
+
+__device__ inline void log_event(const SysDev& S, int& log_count, long long coll, int e, int jj, int ty, double t) {
+    if (S.log && log_count < S.log_cap) {
+        if (lane_id() == 0) {
+            LogRec r;
+            r.coll = coll;
+            r.i = e + 1;
+            r.j = jj + 1;
+            r.type = ty;
+            r.pad = 0;
+            r.t = t;
+            gst(S.log + (log_count), r);
+        }
+        ++log_count;
+    }
+}
+
+// The event loop: each workgroup commits events of its system until the
+// system's coll counter has advanced by inc[k].  With one wavefront per
+// workgroup each round is one event.  With W wavefronts a round pops up to
+// W entries; pair events run side by side, one per wavefront, and the
+// round commits the prefix of them that the serial order would have run
+// the same way (disjoint footprints, no new entry ahead of a later one).
 // A wall-clock budget (ticks of wall_clock64, 0 = none) stops a system early
-// between events, so that one launch over systems of uneven speed ends evenly;
-// flags & kStopAtOutput stops it right after its next output pseudo-event
-// (where the reference writes observables and configurations).
-__global__ __launch_bounds__(64) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc, int nblk_max,
-                                            int maxw, int leaf_cap, long long budget, int flags) {
+// between rounds, so that one launch over systems of uneven speed ends
+// evenly; flags & kStopAtOutput stops it right after its next output
+// pseudo-event (where the reference writes observables and configurations).
+__global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc,
+                                                       int nblk_max, int maxw, int leaf_cap, long long budget, int flags) {
+    const int nw = (int)(blockDim.x >> 6), wave = wave_id(), lane = lane_id();
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
     const SysDev& S = sdev();
-    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, nblk_max, maxw, leaf_cap);
+    if (lane == 0)
+        *reinterpret_cast<Lds*>(smem + kLdOff + wave * kLdStride) = make_lds(S, nblk_max, maxw, leaf_cap, nw, wave);
     __syncthreads();
     const Lds& L = ldsv();
-    const int lane = threadIdx.x;
     const int N = S.N, N3 = N + 3;
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
     if (L.leaf_lds)
-        for (int e = lane; e < N3; e += kW) L.lleaf()[e] = gld(S.leaf + (e));
+        for (int e = (int)threadIdx.x; e < N3; e += (int)blockDim.x) L.lleaf()[e] = gld(S.leaf + (e));
     __syncthreads();
     const DevTables& T = tabs();
-    Sc& c = *L.sc();
-    if (lane == 0) {
+    Sc& c = *L.sc();  // the master scalars on wavefront 0
+    Batch& B = *L.batch();
+    if (threadIdx.x == 0) {
         load_sc(S.st, c);
         c.target = c.coll + inc[blockIdx.x];
     }
     __syncthreads();
-    rescan_all(L, N3);
+    if (wave == 0) rescan_all(L, N3);
+    __syncthreads();
     int log_count = S.st->log_count;
     const long long w0 = wall_clock64();
     bool stop_out = false;
-    while (c.coll < c.target && c.status == 0 && !stop_out) {
-        if (budget > 0 && wall_clock64() - w0 > budget) break;
-        PH_BEGIN(tsel);
-        // next event: min over the superblock minima of (bin, time, index)
-        const Key k = key_min_wave(lane < L.nsup ? L.sup()[lane] : empty_key());
-        if (k.bc == kEmptyBc) {
-            c.status = kErrBin;
-            break;
-        }
-        const int e = key_idx(k);
-        c.old_tfalse = c.tfalse;
-        c.nbin = key_bin(k);
-        c.tfalse = k.t;
-        c.coll++;
-        PH_END(S, kPhSelect, tsel);
-        int jj = -1, ty = 0;
-        if (e < N) {
-            jj = key_p(k);
-            ty = pair_event(S, T, L, c, e, jj, key_ct(k));
-            if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
-        } else if (e == N) {
-            jj = ghost_event(S, T, L, c);
-        } else if (e == N + 1) {
-            sync_event(S, T, L, c);
-        } else {
-            output_event(S, L, c);
-            stop_out = (flags & kStopAtOutput) != 0;
-        }
-        if (S.log && log_count < S.log_cap) {
-            if (lane == 0) {
-                LogRec r;
-                r.coll = c.coll;
-                r.i = e + 1;
-                r.j = jj + 1;
-                r.type = ty;
-                r.pad = 0;
-                r.t = k.t;
-                gst(S.log + (log_count), r);
+    while (true) {
+        if (wave == 0) {
+            PH_BEGIN(tsel);
+            int K = 0;
+            if (c.coll < c.target && c.status == 0 && !stop_out && !(budget > 0 && wall_clock64() - w0 > budget)) {
+                long long left = c.target - c.coll;
+                K = select_batch(L, c, N, N3, (int)min((long long)nw, left));
             }
-            ++log_count;
+            if (lane == 0) {
+                B.K = K;
+                B.cut = K;
+            }
+            PH_END(S, kPhSelect, tsel);
         }
+        if (nw > 1)
+            __syncthreads();
+        else
+            wave_sync();
+        const int K = B.K;
+        if (K == 0) break;
+        if (K == 1) {
+            if (wave == 0) {
+                const Key k = B.key[0];
+                const int e = key_idx(k);
+                int jj = -1, ty = 0;
+                if (e < N) {
+                    jj = key_p(k);
+                    ty = pair_event(S, T, L, c, e, jj, key_ct(k), -1);
+                    if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
+                } else if (e == N) {
+                    jj = ghost_event(S, T, L, c);
+                } else if (e == N + 1) {
+                    sync_event(S, T, L, c);
+                } else {
+                    output_event(S, L, c);
+                    stop_out = (flags & kStopAtOutput) != 0;
+                }
+                log_event(S, log_count, c.coll, e, jj, ty, k.t);
+            }
+            if (nw > 1) __syncthreads();
+            continue;
+        }
+        // a round of K >= 2 pair events, one per wavefront
+        if (wave < K) {
+            const Key k = B.key[wave];
+            pair_event(S, T, L, *L.sc(), key_idx(k), key_p(k), key_ct(k), wave);
+        } else {
+            int nb = 0;
+            bar_to(nb, 3);
+        }
+        const int A = batch_accepted(B);
+        if (wave == 0) {
+            PH_COUNT(S, kPhNRound);
+            PH_ADDN(S, kPhNPopped, K);
+            PH_ADDN(S, kPhNCommit, A);
+        }
+        __syncthreads();
+        // block minima of every committed event's blocks and of the popped
+        // entries that were not committed, then the superblock minima
+        if (wave == 0) {
+            int nbl = B.nbl[0];
+            for (int k = A; k < K; ++k) {
+                if (lane == 0) L.blist()[nbl] = key_idx(B.key[k]) / kBlk;
+                ++nbl;
+            }
+            if (lane == 0) B.nbl[0] = nbl;
+            wave_sync();
+        }
+        if (wave < A) rescan_blocks(L, N3, L.blist(), B.nbl[wave]);
+        __syncthreads();
+        if (wave < A) rescan_sups(L, L.blist(), B.nbl[wave]);
+        if (wave == 0) {
+            // the master takes the committed events in order
+            for (int k = 0; k < A; ++k) {
+                const Key kk = B.key[k];
+                const int ty = B.ty[k];
+                if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
+                log_event(S, log_count, c.coll + k, key_idx(kk), key_p(kk), ty, kk.t);
+            }
+            if (A >= 2) {
+                if (lane == 0) {
+                    for (int k = 1; k < A; ++k)
+                        if (B.st[k] != 0 && c.status == 0) c.status = B.st[k];
+                    c.coll += A - 1;
+                    c.old_tfalse = B.key[A - 2].t;
+                    c.tfalse = B.key[A - 1].t;
+                    c.nbin = key_bin(B.key[A - 1]);
+                }
+            }
+        }
+        __syncthreads();
     }
     __syncthreads();
     if (L.leaf_lds)
-        for (int e2 = lane; e2 < N3; e2 += kW) gst(S.leaf + (e2), L.lleaf()[e2]);
-    if (lane == 0) {
+        for (int e2 = (int)threadIdx.x; e2 < N3; e2 += (int)blockDim.x) gst(S.leaf + (e2), L.lleaf()[e2]);
+    if (threadIdx.x == 0) {
         S.st->log_count = log_count;
         store_sc(S.st, c);
     }
@@ -2635,32 +3018,44 @@ __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const
 int engine_maxw(int cap) { return max((4 + kMaxSpec) * (cap + 3), 768); }
 
 static size_t r16(size_t b) { return (b + 15) / 16 * 16; }
-size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap) {
-    size_t b = r16(sizeof(DevTables)) + r16(sizeof(SysDev)) + r16(sizeof(Lds)) + r16(sizeof(Key) * nblk_max) + r16(sizeof(Key) * kSup) +
-               4 * r16(sizeof(int) * maxw) + r16(sizeof(int) * kMaxBlist) + r16(sizeof(double) * maxw) +
-               r16(sizeof(MinSlot) * (2 + kMaxSpec)) + r16(sizeof(SpecInfo) * kMaxSpec) +
-               r16(sizeof(DownInfo) * kW) + r16(sizeof(Pair)) +
-               r16(sizeof(Sc));
-    b += sizeof(Leaf) * (size_t)leaf_cap;
+// the bytes make_lds lays out before the leaves, for nw wavefronts
+static size_t lds_fixed(int nblk_max, int maxw, int nw) {
+    size_t b = r16(sizeof(DevTables)) + r16(sizeof(SysDev)) + (size_t)kLdStride * kMaxWaves +
+               r16(sizeof(Key) * nblk_max) + r16(sizeof(Key) * kSup) + r16(sizeof(Sc)) + r16(sizeof(Batch));
+    const size_t common = 3 * r16(sizeof(int) * maxw) + r16(sizeof(int) * kMaxBlist) + r16(sizeof(double) * maxw) +
+                          r16(sizeof(MinSlot) * (2 + kMaxSpec)) + r16(sizeof(SpecInfo) * kMaxSpec) +
+                          r16(sizeof(DownInfo) * kW) + r16(sizeof(Pair));
+    b += common + r16(sizeof(int) * maxw);                                          // wavefront 0
+    b += (size_t)(nw - 1) * (common + r16(sizeof(int) * 16) + r16(sizeof(Sc)));   // the others
     return b;
+}
+size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap, int nw) {
+    return lds_fixed(nblk_max, maxw, nw) + sizeof(Leaf) * (size_t)leaf_cap;
 }
 static int nblk_for(int n) { return (n + 3 + kBlk - 1) / kBlk; }
 
 hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxw, hipStream_t s) {
-    const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0);
+    const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0, 1);
     hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW), lds, s, d_sys, d_ids, nblk_for(max_n), maxw);
     return hipGetLastError();
 }
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s, long long budget_ticks, int flags) {
+                      size_t lds_limit, hipStream_t s, long long budget_ticks, int flags, int waves) {
     const int nblk = nblk_for(max_n);
-    const size_t base = engine_lds_bytes(nblk, maxw, 0);
+    int nw = waves < 1 ? 1 : (waves > kMaxWaves ? kMaxWaves : waves);
+    while (nw > 1 && engine_lds_bytes(nblk, maxw, 0, nw) > lds_limit) --nw;
+    const size_t base = engine_lds_bytes(nblk, maxw, 0, nw);
     int leaf_cap = 0;
     if (lds_limit > base) leaf_cap = (int)((lds_limit - base) / sizeof(Leaf));
     if (leaf_cap > max_n + 3) leaf_cap = max_n + 3;
-    const size_t lds = engine_lds_bytes(nblk, maxw, leaf_cap);
-    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap, budget_ticks, flags);
+    const size_t lds = engine_lds_bytes(nblk, maxw, leaf_cap, nw);
+    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap, budget_ticks, flags);
     return hipGetLastError();
+}
+int engine_max_waves(int max_n, int maxw, size_t lds_limit) {
+    int nw = kMaxWaves;
+    while (nw > 1 && engine_lds_bytes(nblk_for(max_n), maxw, 0, nw) > lds_limit) --nw;
+    return nw;
 }
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s) {

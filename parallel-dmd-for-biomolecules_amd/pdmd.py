@@ -111,6 +111,7 @@ EXPORTS = [
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
     "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
+    "pdmd_set_waves",
 ]
 
 _lib = None
@@ -136,6 +137,7 @@ def lib():
         L.pdmd_advance_async.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
         L.pdmd_wait.argtypes = [ctypes.c_void_p]
         L.pdmd_last_launch_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.pdmd_set_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.pdmd_download.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
         L.pdmd_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]
         L.pdmd_log_enable.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
@@ -213,11 +215,17 @@ def pair_codes(system, a, b, species=None):
 class Engine:
     """A device context holding resident systems (pdmd_ctx)."""
 
-    def __init__(self, param_dir=PARAM_DIR):
+    def __init__(self, param_dir=PARAM_DIR, waves=1):
         self._ctx = ctypes.c_void_p()
         _check(lib().pdmd_ctx_create(param_dir.encode(), ctypes.byref(self._ctx)))
         self._keep = {}
         self.nbeads = {}
+        if waves != 1:
+            self.set_waves(waves)
+
+    def set_waves(self, waves):
+        """Wavefronts per system for later advances (pdmd_set_waves)."""
+        _check(lib().pdmd_set_waves(self._ctx, int(waves)))
 
     def close(self):
         if self._ctx:
