@@ -45,11 +45,14 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(th
 // 12 reschedule down-set scan, 13 rebuild count, 14 sync count; list
 // rebuild steps: 16 cells and histogram, 17 scan, 18 scatter, 19 sorted
 // copies, 20 neighbour search; multi-wavefront rounds: 21 rounds of two or
-// more pair events, 22 entries popped in them, 23 entries committed.
+// more pair events, 22 entries popped in them, 23 entries committed, 24
+// footprint marks up to the first round barrier, 25 footprint checks up to
+// the second, 26 new keys up to the third, 27 the round's rescans and merge.
 enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
        kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
        kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20,
-       kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23 };
+       kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23, kPhBMark = 24, kPhBCheck = 25, kPhBKeys = 26,
+       kPhBTail = 27 };
 #ifdef PDMD_PHASE_TIMERS
 #define PH_BEGIN(v) long long v = clock64()
 #define PH_END(S, k, v)                                                                                  \
@@ -399,6 +402,14 @@ __device__ inline void wave_sync_mem() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
 }
+// Barrier of every wavefront of the system's workgroup, for the steps that
+// all of them run together (list rebuilds, full rescans).
+__device__ inline void wg_sync(int nw) {
+    if (nw > 1)
+        __syncthreads();
+    else
+        wave_sync_mem();
+}
 
 // DPP lane moves: CTRL row_shr:n = 0x110+n, row_bcast:15 = 0x142,
 // row_bcast:31 = 0x143; lanes without a valid source keep their own value
@@ -499,22 +510,18 @@ __device__ inline void key_step(Key& k) {
     o.ip = (unsigned)dpp_i<CTRL, RM>((int)k.ip);
     if (key_less(o, k)) k = o;
 }
-// minimum over each row of 16 lanes, in the row's last lane
-__device__ inline void key_min_rows(Key& k) {
+// exact minimum over each row of 16 lanes, in the row's last lane
+__device__ inline void key_min_rows_exact(Key& k) {
     key_step<0x111, 0xf>(k);
     key_step<0x112, 0xf>(k);
     key_step<0x114, 0xf>(k);
     key_step<0x118, 0xf>(k);
 }
-// minimum over the wave, returned to every lane
-__device__ inline Key key_min_wave(Key k) {
-    key_min_rows(k);
-    key_step<0x142, 0xa>(k);
-    key_step<0x143, 0xc>(k);
+__device__ inline Key readlane_key(const Key& k, int l) {
     Key r;
-    r.t = readlane_d(k.t, 63);
-    r.bc = __builtin_amdgcn_readlane(k.bc, 63);
-    r.ip = (unsigned)__builtin_amdgcn_readlane((int)k.ip, 63);
+    r.t = readlane_d(k.t, l);
+    r.bc = __builtin_amdgcn_readlane(k.bc, l);
+    r.ip = (unsigned)__builtin_amdgcn_readlane((int)k.ip, l);
     return r;
 }
 
@@ -586,7 +593,73 @@ struct Batch {
     int st[kMaxWaves];    // status after each event
     int K;                // entries popped this round (0: stop)
     int cut;              // first wavefront whose event is not committed
+    int rebuild;          // the round's sync event rebuilds the lists
 };
+// Order proxy of a calendar key: the bin in the top 12 bits, then the top
+// 52 bits of the ordered time.  Proxies order keys as key_less does except
+// that distinct keys may share one; the reductions below take the minimum
+// proxy with 64-bit lane moves and settle shared proxies with key_less.
+__device__ inline unsigned long long key_proxy(const Key& k) {
+    return k.bc == kEmptyBc ? ~0ull
+                            : (((unsigned long long)(unsigned)(k.bc >> 8) << 52) | (t_order(k.t) >> 12));
+}
+template <int CTRL, int RM>
+__device__ inline unsigned long long dpp_u64(unsigned long long v) {
+    const int lo = dpp_i<CTRL, RM>((int)v), hi = dpp_i<CTRL, RM>((int)(v >> 32));
+    return (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32);
+}
+template <int CTRL, int RM>
+__device__ inline void umin_step(unsigned long long& v) {
+    const unsigned long long o = dpp_u64<CTRL, RM>(v);
+    v = o < v ? o : v;
+}
+// minimum over the wave, returned to every lane
+__device__ inline Key key_min_wave(Key k) {
+    const unsigned long long p = key_proxy(k);
+    unsigned long long m = p;
+    umin_step<0x111, 0xf>(m);
+    umin_step<0x112, 0xf>(m);
+    umin_step<0x114, 0xf>(m);
+    umin_step<0x118, 0xf>(m);
+    umin_step<0x142, 0xa>(m);
+    umin_step<0x143, 0xc>(m);
+    const unsigned long long mn = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)m, 63) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(m >> 32), 63) << 32);
+    if (mn == ~0ull) return empty_key();
+    const unsigned long long tie = __ballot(p == mn);
+    Key r = readlane_key(k, __ffsll((long long)tie) - 1);
+    unsigned long long rest = tie & (tie - 1);
+    while (rest) {
+        const Key o = readlane_key(k, __ffsll((long long)rest) - 1);
+        rest &= rest - 1;
+        if (key_less(o, r)) r = o;
+    }
+    return r;
+}
+// Minimum over each row of 16 lanes.  Returns true on the lane that holds
+// its row's minimum in k afterwards (the row's last lane when the row is
+// empty or when shared proxies need the exact scan).
+__device__ inline bool key_min_rows(Key& k) {
+    const int lane = lane_id();
+    const unsigned long long p = key_proxy(k);
+    unsigned long long m = p;
+    umin_step<0x111, 0xf>(m);
+    umin_step<0x112, 0xf>(m);
+    umin_step<0x114, 0xf>(m);
+    umin_step<0x118, 0xf>(m);
+    const unsigned long long rm = dpp_u64<0x15F, 0xf>(m);  // the row's last lane, to the whole row
+    const unsigned long long tie = __ballot(p == rm && rm != ~0ull);
+    const unsigned rb = (unsigned)(tie >> (lane & 48)) & 0xFFFFu;
+    if (__ballot(__popc(rb) > 1)) {
+        key_min_rows_exact(k);
+        return (lane & 15) == 15;
+    }
+    if (rm == ~0ull) {
+        k = empty_key();
+        return (lane & 15) == 15;
+    }
+    return p == rm;
+}
 
 // LDS views, one set per wavefront: the calendar tree, the master scalars
 // and the round record are shared, the work lists are the wavefront's own
@@ -630,15 +703,22 @@ __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
 __device__ void rescan_blocks(const Lds&, int N3, const int* list, int nb) {
     const Lds& L = ldsv();
 
-    const int lane = lane_id();
-    for (int base = 0; base < nb; base += 4) {
-        const int q = base + (lane >> 4), k = lane & 15;
-        const int b = q < nb ? list[q] : -1;
-        Key e = empty_key();
-        const int idx = b * kBlk + k;
-        if (b >= 0 && idx < N3) e = leaf_key(leaf_get(L, idx), idx);
-        key_min_rows(e);
-        if (k == 15 && b >= 0) L.blk()[b] = e;
+    const int lane = lane_id(), k = lane & 15;
+    // 16 blocks per step: the leaf loads of four passes are in flight together
+    for (int base = 0; base < nb; base += 16) {
+        Key e[4];
+        int bq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = base + 4 * u + (lane >> 4);
+            bq[u] = q < nb ? list[q] : -1;
+            const int idx = bq[u] * kBlk + k;
+            e[u] = empty_key();
+            if (bq[u] >= 0 && idx < N3) e[u] = leaf_key(leaf_get(L, idx), idx);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (base + 4 * u < nb && key_min_rows(e[u]) && bq[u] >= 0) L.blk()[bq[u]] = e[u];
     }
 }
 // superblock minima over the superblocks of the listed blocks (at most 64),
@@ -671,25 +751,25 @@ __device__ void rescan_block_list(const Lds&, int N3, int nb) {
     rescan_sups(L, L.blist(), nb);
     wave_sync();
 }
+// every block and superblock minimum; all wavefronts of the system take part
 __device__ void rescan_all(const Lds&, int N3) {
     const Lds& L = ldsv();
 
-    const int lane = lane_id();
-    for (int base = 0; base < L.nblk; base += 4) {
+    const int lane = lane_id(), wave = wave_id(), nw = L.nw;
+    for (int base = 4 * wave; base < L.nblk; base += 4 * nw) {
         const int b = base + (lane >> 4), k = lane & 15;
         Key e = empty_key();
         const int idx = b * kBlk + k;
         if (b < L.nblk && idx < N3) e = leaf_key(leaf_get(L, idx), idx);
-        key_min_rows(e);
-        if (k == 15 && b < L.nblk) L.blk()[b] = e;
+        if (key_min_rows(e) && b < L.nblk) L.blk()[b] = e;
     }
-    wave_sync_mem();
-    for (int s = 0; s < L.nsup; ++s) {
+    wg_sync(nw);
+    for (int s = wave; s < L.nsup; s += nw) {
         const int b = s * kSup + lane;
         const Key e = key_min_wave(b < L.nblk ? L.blk()[b] : empty_key());
         if (lane == 0) L.sup()[s] = e;
     }
-    wave_sync_mem();
+    wg_sync(nw);
 }
 
 // ------------------------------------------------------------------------
@@ -1379,6 +1459,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     //      event reads the state before this one's writes (they wait for the
     //      round's last barrier), as in the serial order.
     if (batch) {
+        PH_RESET(tph);
         Batch& B = *L.batch();
         const unsigned sbit = 1u << bk, lbit = 1u << (8 + bk);
         const int sp = shoulder_bead(S, P, lane);
@@ -1387,6 +1468,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit);
         if (bk == 0 && sp >= 0) mark_or(S.mark + sp, sbit);
         bar_to(nb, 1);
+        PH_END(S, kPhBMark, tph);
         if (bk >= 1) {
             const unsigned lowS = sbit - 1u, lowL = lowS << 8;
             bool bad = false;
@@ -1409,6 +1491,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             if (__ballot(bad) && lane == 0) atomicMin(&B.cut, bk);
         }
         bar_to(nb, 2);
+        PH_END(S, kPhBCheck, tph);
         // every wavefront has checked: clear this one's marks
         if (dent >= 0) mark_clear(S.mark + dent);
         if (lane == 0) mark_clear(S.mark + i);
@@ -1650,6 +1733,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         // report the least new calendar key and the status, then commit
         // only within the round's accepted prefix
         Batch& B = *L.batch();
+        PH_END(S, kPhDown, tph);
         Key kn = wr ? leaf_key(lf, l) : empty_key();
         if (lane == 0) {
             const Key ka = leaf_key(la, i);
@@ -1665,6 +1749,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             B.st[bk] = c.status;
         }
         bar_to(nb, 3);
+        PH_END(S, kPhBKeys, tph);
         if (bk >= batch_accepted(B)) return;
         if (bk >= 1 && lane == 0) {
             gst(S.rec + (i), P.ri);
@@ -2211,11 +2296,12 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     const SysDev& S = sdev();
 
     const DevTables& T = tabs();
-    const int lane = lane_id();
+    const int lane = lane_id(), wave = wave_id(), nw = L.nw;
+    const int tid = (int)threadIdx.x, nthr = kW * nw;
     const int N = S.N, M = S.M, Mq = S.Mq, NC = S.ncoarse, cap = S.cap;
     PH_BEGIN(trl);
 #pragma unroll 4
-    for (int k = lane; k < N; k += kW) {
+    for (int k = tid; k < N; k += nthr) {
         const Rec b = gld(S.rec + (k));
         int cx = (int)((b.x + S.half) / S.width);
         int cy = (int)((b.y + S.half) / S.width);
@@ -2227,17 +2313,17 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         const int qx = min(cx >> 1, Mq - 1), qy = min(cy >> 1, Mq - 1), qz = min(cz >> 1, Mq - 1);
         gs1(S.ckey + (k), qx + Mq * (qy + Mq * qz));
     }
-    for (int q = lane; q < NC; q += kW) {
+    for (int q = tid; q < NC; q += nthr) {
         gs1(S.cstart + (q), 0);
         gs1(S.crun + (q), 0);
     }
-    wave_sync_mem();
-    for (int k = lane; k < N; k += kW) atomicAdd(&S.cstart[gl1(S.ckey + k)], 1);
-    wave_sync_mem();
+    wg_sync(nw);
+    for (int k = tid; k < N; k += nthr) atomicAdd(&S.cstart[gl1(S.ckey + k)], 1);
+    wg_sync(nw);
     PH_END(S, kPhLCells, trl);
-    // exclusive scan of the cell counts
+    // exclusive scan of the cell counts (wavefront 0)
     int carry = 0;
-    for (int base = 0; base < NC; base += kW) {
+    for (int base = 0; wave == 0 && base < NC; base += kW) {
         const int q = base + lane;
         const int v = q < NC ? gl1(S.cstart + (q)) : 0;
         int incl = v;
@@ -2250,11 +2336,12 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         if (q < NC) gs1(S.cstart + (q), carry + incl - v);
         carry += __builtin_amdgcn_readlane(incl, kW - 1);
     }
-    if (lane == 0) gs1(S.cstart + (NC), carry);
-    wave_sync_mem();
+    if (wave == 0 && lane == 0) gs1(S.cstart + (NC), carry);
+    wg_sync(nw);
     PH_END(S, kPhLScan, trl);
-    // stable scatter in bead order: rank among the lower lanes of the same cell
-    for (int base = 0; base < N; base += kW) {
+    // stable scatter in bead order (wavefront 0): rank among the lower lanes
+    // of the same cell
+    for (int base = 0; wave == 0 && base < N; base += kW) {
         const int k = base + lane;
         const int key = k < N ? gl1(S.ckey + (k)) : -1;
         int rank = 0, cnt = 0;
@@ -2274,22 +2361,24 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         }
         wave_sync_mem();
     }
+    wg_sync(nw);
     PH_END(S, kPhLScatter, trl);
     // positions, topology and fine cells in sorted order
 #pragma unroll 4
-    for (int p2 = lane; p2 < N; p2 += kW) {
+    for (int p2 = tid; p2 < N; p2 += nthr) {
         const int jb = gl1(S.csorted + (p2));
         const Rec b = gld(S.rec + (jb));
         gst(S.spos + (p2), make_double4(b.x, b.y, b.z, 0.0));
         gst(S.sinfo + (p2), make_int4(jb, b.topo, gl1(S.cfine + jb), 0));
     }
-    wave_sync_mem();
+    wg_sync(nw);
     PH_END(S, kPhLCopy, trl);
     int mx = 0, status = 0;
     const int noff = Mq > 2 ? 3 : Mq;  // distinct coarse offsets {0, +1, -1}
-    // per lane, its bead's non-empty neighbour cell ranges, [t][lane] in LDS
+    // per lane, its bead's non-empty neighbour cell ranges, [t][lane] in the
+    // wavefront's own work area; wavefronts take 64-bead chunks in turn
     int2* rng = reinterpret_cast<int2*>(smem + L.o_wm);
-    for (int i0 = 0; i0 < N; i0 += kW) {
+    for (int i0 = wave * kW; i0 < N; i0 += nw * kW) {
         const int i = i0 + lane;
         const bool act = i < N;
         const int ii = act ? i : 0;
@@ -2401,24 +2490,25 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     status = wave_status(status);
     if (status && !c.status) c.status = status;
     mx = wave_imax(mx);
-    if (lane == 0 && mx > S.st->max_nb_seen) S.st->max_nb_seen = mx;
-    wave_sync_mem();
+    if (lane == 0) atomicMax(&S.st->max_nb_seen, (long long)mx);
+    wg_sync(nw);
 }
 
-// events.f:11-127 with tfalse = 0: every bead's next event, then the calendar
+// events.f:11-127 with tfalse = 0: every bead's next event, then the
+// calendar; all wavefronts of the system take part (the master scalars'
+// scan position is at bin 1 already)
 __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
 
     const DevTables& T = tabs();
-    Sc& c = *L.sc();
+    Sc& c = *L.msc();
 
-    const int lane = lane_id();
+    const int tid = (int)threadIdx.x, nthr = kW * L.nw;
     const int N = S.N, cap = S.cap;
     int status = 0;
-    c.nbin = 1;
-    for (int i = lane; i < N; i += kW) {
+    for (int i = tid; i < N; i += nthr) {
         const Rec bi = gld(S.rec + (i));
         const double2 bwi = gld(S.bw + (i));
         double ti = c.interval_max + kLt;
@@ -2464,30 +2554,29 @@ __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
         lf.meta = make_meta(calc_bin(c, ti, false, status), ci);
         leaf_set(L, i, lf);
     }
-    for (int e = N + lane; e < N + 3; e += kW) {
+    for (int e = N + tid; e < N + 3; e += nthr) {
         Leaf lf = leaf_get(L, e);
         lf.meta = make_meta(calc_bin(c, lf.tim, false, status), -2);
         leaf_set(L, e, lf);
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
-    if (!L.leaf_lds)
-        wave_sync_mem();
-    else
-        wave_sync();
+    wg_sync(L.nw);
     rescan_all(L, N + 3);
 }
 
+// Wrap, lists and full reschedule; all wavefronts of the system take part.
 __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
 
     const DevTables& T = tabs();
-    Sc& c = *L.sc();
+    Sc& c = *L.msc();
 
-    const int lane = lane_id();
-    for (int k = lane; k < S.N; k += kW) {
+    const int tid = (int)threadIdx.x, nthr = kW * L.nw;
+    if (tid == 0) c.nbin = 1;
+    for (int k = tid; k < S.N; k += nthr) {
         Rec b = gld(S.rec + (k));
         b.x = b.x - round(b.x);
         b.y = b.y - round(b.y);
@@ -2495,7 +2584,7 @@ __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
         gst(S.rec + (k), b);
         gst(S.old_r + (k), make_double4(b.x, b.y, b.z, 0.0));
     }
-    wave_sync_mem();
+    wg_sync(L.nw);
     PH_BEGIN(trb);
     rebuild_lists(S, L, c);
     PH_END(S, kPhLists, trb);
@@ -2504,8 +2593,11 @@ __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
     PH_COUNT(S, kPhNRebuild);
 }
 
-// sync pseudo-event, main.F90:1126-1187
-__device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
+// sync pseudo-event, main.F90:1126-1187, in two parts around the list
+// rebuild that all wavefronts run: sync_begin (wavefront 0) advances the
+// free flight, shifts the calendar and returns whether the lists are
+// rebuilt; sync_finish (wavefront 0) schedules the next sync.
+__device__ bool sync_begin(const SysDev&, const DevTables&, const Lds&, Sc&) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
@@ -2563,8 +2655,7 @@ __device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
         c.tbin_off = 0.0;
         c.nupdates++;
         PH_END(S, kPhSync, tsy);
-        rebuild(S, T, L, c);
-        PH_RESET(tsy);
+        return true;
     } else {
         // every entry moved by the same -tf: the cached block and superblock
         // minima are shifted exactly as their leaves were
@@ -2573,6 +2664,19 @@ __device__ void sync_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
         if (lane < L.nsup && L.sup()[lane].bc != kEmptyBc) L.sup()[lane].t = L.sup()[lane].t - tf;
         wave_sync_mem();
     }
+    PH_END(S, kPhSync, tsy);
+    return false;
+}
+__device__ void sync_finish(const SysDev&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
+    Sc& c = *L.sc();
+
+    const int lane = lane_id();
+    const int N = S.N;
+    PH_BEGIN(tsy);
     int status = 0;
     const double t2 = c.interval * 0.999;
     if (lane == 0) {
@@ -2804,8 +2908,7 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
             for (int m = 0; m < K; ++m) popped |= key_idx(B.key[m]) == idx;
             if (!popped) x = leaf_key(leaf_get(L, idx), idx);
         }
-        key_min_rows(x);
-        if (lane == 15) L.blk()[b] = x;
+        if (key_min_rows(x) && lane < kBlk) L.blk()[b] = x;
         wave_sync();
         const int bb = sp * kSup + lane;
         const Key y = key_min_wave(bb < L.nblk ? L.blk()[bb] : empty_key());
@@ -2814,20 +2917,21 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
     }
     wave_sync();
     if (K == 0) return 0;
+    const long long cw = lane < (int)(sizeof(Sc) / 8) ? reinterpret_cast<const long long*>(&c)[lane] : 0;
     for (int k = 1; k < K; ++k) {
         const Lds& Lk = *reinterpret_cast<const Lds*>(smem + kLdOff + k * kLdStride);
-        Sc* ck = Lk.sc();
-        if (lane < (int)(sizeof(Sc) / 8))
-            reinterpret_cast<long long*>(ck)[lane] = reinterpret_cast<const long long*>(&c)[lane];
-        wave_sync();
-        if (lane == 0) {
+        if (lane < (int)(sizeof(Sc) / 8)) reinterpret_cast<long long*>(Lk.sc())[lane] = cw;
+    }
+    wave_sync();
+    if (lane == 0)
+        for (int k = 1; k < K; ++k) {
+            Sc* ck = reinterpret_cast<const Lds*>(smem + kLdOff + k * kLdStride)->sc();
             ck->old_tfalse = B.key[k - 1].t;
             ck->tfalse = B.key[k].t;
             ck->nbin = key_bin(B.key[k]);
             ck->coll = c.coll + 1 + k;
             ck->status = 0;
         }
-    }
     wave_sync();
     const Key k0 = B.key[0];
     c.old_tfalse = c.tfalse;
@@ -2886,8 +2990,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
         c.target = c.coll + inc[blockIdx.x];
     }
     __syncthreads();
-    if (wave == 0) rescan_all(L, N3);
-    __syncthreads();
+    rescan_all(L, N3);
     int log_count = S.st->log_count;
     const long long w0 = wall_clock64();
     bool stop_out = false;
@@ -2912,9 +3015,24 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
         const int K = B.K;
         if (K == 0) break;
         if (K == 1) {
-            if (wave == 0) {
-                const Key k = B.key[0];
-                const int e = key_idx(k);
+            const Key k = B.key[0];
+            const int e = key_idx(k);
+            if (e == N + 1) {
+                // the sync event; every wavefront takes part in a list rebuild
+                if (wave == 0) {
+                    const bool rb = sync_begin(S, T, L, c);
+                    if (lane == 0) B.rebuild = rb ? 1 : 0;
+                }
+                if (nw > 1)
+                    __syncthreads();
+                else
+                    wave_sync();
+                if (B.rebuild) rebuild(S, T, L, c);
+                if (wave == 0) {
+                    sync_finish(S, L, c);
+                    log_event(S, log_count, c.coll, e, -1, 0, k.t);
+                }
+            } else if (wave == 0) {
                 int jj = -1, ty = 0;
                 if (e < N) {
                     jj = key_p(k);
@@ -2922,8 +3040,6 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
                     if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
                 } else if (e == N) {
                     jj = ghost_event(S, T, L, c);
-                } else if (e == N + 1) {
-                    sync_event(S, T, L, c);
                 } else {
                     output_event(S, L, c);
                     stop_out = (flags & kStopAtOutput) != 0;
@@ -2941,6 +3057,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             int nb = 0;
             bar_to(nb, 3);
         }
+        PH_BEGIN(ttail);
         const int A = batch_accepted(B);
         if (wave == 0) {
             PH_COUNT(S, kPhNRound);
@@ -2982,6 +3099,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             }
         }
         __syncthreads();
+        PH_END(S, kPhBTail, ttail);
     }
     __syncthreads();
     if (L.leaf_lds)

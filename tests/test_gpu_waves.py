@@ -41,6 +41,7 @@ def test_single_system_matches_serial(param_dir, waves, canon):
         # uneven chunks: rounds end at every event target
         for chunk in (1, 7, 992, 5000, 6000):
             eng.advance([sid], [chunk])
+        assert eng.stats(sid).nupdates >= 1, "no list rebuild inside the compared window"
         _compare(eng, sid, sysd, 77, n, param_dir, canon)
     finally:
         eng.close()
