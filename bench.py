@@ -6,8 +6,9 @@ Workload (default): 64 uneven PRIME20 systems (chain counts log-uniform in
 rho* in [0.005,0.02], T* in [0.15,0.50], random sequences and coils from a
 seed).  A step advances every system by --per-step committed collisions.
 
-  * batched: all systems in one persistent launch (one wavefront each)
-  * plain:   the same systems advanced one at a time (one launch each)
+  * batched: all systems in one persistent launch (one workgroup each, W
+             wavefronts per system picked by the autotuner unless --waves)
+  * plain:   the same systems advanced one at a time (one launch each, same W)
   * cpu_baseline: the CPU oracle on a bounded sample of the same systems,
     one system per host core
 
@@ -77,7 +78,23 @@ def cpu_baseline(systems, seeds, per_sys, param_dir, cores):
     return ncoll / busy, len(work), wall, ncoll
 
 
-def serve_run(systems, seeds, args):
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_bytes_per_entry():
+    """HBM bytes per committed entry of k_run from the FETCH_SIZE/WRITE_SIZE
+    counter passes summarised in profiles/ (None if absent)."""
+    path = os.path.join(PROFILES, "k_run_uneven64_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("bytes_per_entry:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def serve_run(systems, seeds, args, waves):
     """Poisson trajectory requests against the continuous-batching server.
 
     Requests arrive at rate --serve-rate for --serve-seconds; each picks one of
@@ -91,7 +108,7 @@ def serve_run(systems, seeds, args):
     arr = np.sort(rng.uniform(0.0, args.serve_seconds, n_req))
     pick = rng.integers(0, len(systems), n_req)
     ncoll = np.exp(rng.uniform(math.log(args.serve_min), math.log(args.serve_max), n_req)).astype(np.int64)
-    eng = pdmd.Engine()
+    eng = pdmd.Engine(waves=waves)
     srv = pdmd.Server(eng, mem_budget_bytes=int(args.serve_budget_gib * (1 << 30)), quantum=args.serve_quantum,
                       slice_ms=args.serve_slice_ms)
     t0 = time.perf_counter()
@@ -146,6 +163,8 @@ def main():
     ap.add_argument("--nsys", type=int, default=64)
     ap.add_argument("--plain-per-sys", type=int, default=0, help="collisions per system in the plain run (0 = per-step)")
     ap.add_argument("--cpu-per-sys", type=int, default=3000000)
+    ap.add_argument("--waves", type=int, default=0, help="wavefronts per system (0: autotune)")
+    ap.add_argument("--tune-events", type=int, default=3000, help="entries per system per autotuner trial")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -186,9 +205,16 @@ def main():
     setup_s = time.perf_counter() - t0
     beads = [eng.nbeads[i] for i in ids]
 
-    # batched: warm-up, then K timed steps, each one launch over all systems
+    # batched: wavefronts per system (given, or autotuned after the warm-up),
+    # warm-up, then K timed steps, each one launch over all systems
+    tune = None
+    if args.waves > 0:
+        eng.set_waves(args.waves)
+        waves = args.waves
     for _ in range(args.warmup):
         eng.advance(ids, [args.per_step] * len(ids))
+    if args.waves <= 0:
+        waves, tune = eng.autotune_waves(ids, args.tune_events)
     coll0 = sum(eng.stats(i).coll for i in ids)
     launch_ms = []
     t0 = time.perf_counter()
@@ -207,6 +233,7 @@ def main():
     ev_per_launch = args.per_step * len(ids)
     avg_launch_s = float(np.mean(launch_ms)) / 1e3
     achieved = ev_per_launch * bpe / avg_launch_s / 1e9
+    bpe_meas = measured_bytes_per_entry()
 
     # plain: the same systems one at a time
     plain = None
@@ -221,7 +248,7 @@ def main():
 
     serve = None
     if not args.no_serve:
-        serve = serve_run(systems, seeds, args)
+        serve = serve_run(systems, seeds, args, waves)
 
     cpu = None
     if not args.no_cpu:
@@ -252,7 +279,9 @@ def main():
             "beads_min": int(min(beads)),
             "beads_max": int(max(beads)),
             "collisions_per_system_per_step": args.per_step,
-            "mode": "batched: one persistent launch, one wavefront per system",
+            "mode": f"batched: one persistent launch, one workgroup of {waves} wavefronts per system",
+            "waves_per_system": waves,
+            "autotune_rates": None if tune is None else {str(k): round(v, 1) for k, v in tune.items()},
         },
         "plain_collisions_per_s": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(value / plain, 2),
@@ -265,7 +294,8 @@ def main():
             "peak": 8000.0,
             "unit": "GB/s",
             "frac": achieved / 8000.0,
-            "traffic": None,
+            "traffic": None if bpe_meas is None else round(bpe_meas * ev_per_launch),
+            "traffic_per_entry": bpe_meas,
             "kernel": "k_run",
             "bytes_per_event": round(bpe, 1),
             "k_up": round(k_up, 2),

@@ -95,7 +95,8 @@ int pdmd_nbeads(pdmd_ctx* ctx, int32_t sys_id);
 int64_t pdmd_system_bytes(const pdmd_system_desc* desc);
 
 /* Commit nevents[k] more calendar entries of each listed system, all
- * systems concurrently in one launch (one wavefront per system), in the
+ * systems concurrently in one launch (one workgroup per system, of the
+ * wavefronts set by pdmd_set_waves / pdmd_autotune_waves), in the
  * reference's serial event order.  Replaces the event loop
  * main.F90:484-1258 (dispatch, commit, partial_events, calendar,
  * ghost/sync/output pseudo-events).  Blocks until done. */
@@ -125,6 +126,14 @@ int pdmd_advance_to_output(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, 
  * whose master/worker speculation (main.F90:484-1258) this replaces.
  * Default 1. */
 int pdmd_set_waves(pdmd_ctx* ctx, int32_t waves_per_system);
+/* Autotuner of the wavefronts per system: advances the listed systems by
+ * probe_events entries each at 1, 2, 4 and 8 wavefronts (those that fit in
+ * LDS), one launch each, and keeps the fastest for later advances.  The
+ * probe entries are committed like any others (the event sequence does not
+ * depend on the setting).  waves_out gets the choice, rates_out[4] (may be
+ * NULL) the committed entries per second of each trial (0 if skipped). */
+int pdmd_autotune_waves(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, int64_t probe_events,
+                        int32_t* waves_out, double* rates_out);
 /* Device time of the most recent pdmd_advance* launch (HIP events on the
  * context's stream), in milliseconds. */
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms);

@@ -594,6 +594,42 @@ int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_t*
     return 0;
 }
 
+int pdmd_autotune_waves(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, int64_t probe_events, int32_t* waves_out,
+                        double* rates_out) {
+    if (!ctx || nsys <= 0 || !ids || probe_events <= 0) return fail("pdmd_autotune_waves: bad arguments");
+    int max_n = 0, max_cap = 0;
+    for (int k = 0; k < nsys; ++k) {
+        HostSys* H = get_sys(ctx, ids[k]);
+        if (!H) return -1;
+        max_n = std::max(max_n, H->N);
+        max_cap = std::max(max_cap, H->cap);
+    }
+    const int top = engine_max_waves(max_n, engine_maxw(max_cap), ctx->lds_limit);
+    const int cand[4] = {1, 2, 4, 8};
+    std::vector<int64_t> nev(nsys, probe_events);
+    int best = 1;
+    double best_rate = -1.0;
+    for (int q = 0; q < 4; ++q) {
+        if (rates_out) rates_out[q] = 0.0;
+        if (cand[q] > top) continue;
+        ctx->waves = cand[q];
+        int e = pdmd_advance(ctx, nsys, ids, nev.data());
+        if (e) return e;
+        float ms = 0.0f;
+        e = pdmd_last_launch_ms(ctx, &ms);
+        if (e) return e;
+        const double rate = (double)probe_events * nsys / std::max(1e-3, (double)ms) * 1e3;
+        if (rates_out) rates_out[q] = rate;
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = cand[q];
+        }
+    }
+    ctx->waves = best;
+    if (waves_out) *waves_out = best;
+    return 0;
+}
+
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms) {
     if (!ctx || !ms) return fail("pdmd_last_launch_ms: null argument");
     if (!ctx->timed) return fail("pdmd_last_launch_ms: no launch yet");

@@ -1,18 +1,23 @@
 // pdmd_engine.hip -- gfx950 kernels of the PRIME20 discontinuous-MD event
-// loop.  One wavefront (64 lanes) owns one system for a whole launch and
-// commits its events in the reference's serial order; the wave
-// parallelises the work inside each event:
-//   * every collision-time prediction an event triggers (full recomputes
-//     of the colliding pair, single-pair updates of their down neighbours)
-//     is issued as one batch of 64-byte gathers, one lane per pair, FP64 in
-//     the reference's evaluation order (no contraction),
-//   * the calendar is a three-level min tree (16-entry leaf blocks, 64-block
+// loop.  One workgroup of W wavefronts (64 lanes each) owns one system for a
+// whole launch and commits its events in the reference's serial order.
+//   * Each round pops up to W calendar entries; the pair events among them
+//     run side by side, one per wavefront, and the round commits the prefix
+//     the serial order would have run the same way (per-bead footprint marks
+//     in HBM, no new calendar key ahead of a later popped one).
+//   * Inside an event the wavefront issues every collision-time prediction
+//     the event triggers (full recomputes of the colliding pair and of the
+//     down beads that pointed at it, single-pair updates of the down set) as
+//     one batch of 64-byte gathers, one lane per pair, FP64 in the
+//     reference's evaluation order (no contraction).
+//   * The calendar is a three-level min tree (16-entry leaf blocks, 64-block
 //     superblocks) whose block/superblock minima carry partner and type, so
-//     the next event comes out of one LDS min-reduction; leaves live in LDS
-//     when the system fits, else in HBM,
-//   * neighbour-list rebuilds (coarse-cell counting sort + per-bead scan),
-//     free flight and displacement checks at the sync event.
-// Many systems in one launch run concurrently, one wavefront each.
+//     the next entry comes out of one LDS min-reduction; leaves live in LDS
+//     when the system fits, else in HBM.
+//   * Neighbour-list rebuilds (coarse-cell counting sort + per-bead search),
+//     free flight and displacement checks at the sync event run on all W
+//     wavefronts.
+// Many systems in one launch run concurrently, one workgroup each.
 #include <hip/hip_runtime.h>
 
 #include <climits>

@@ -22,6 +22,7 @@
 !     seed = 1              ! project RNG seed (replaces IFPORT srand/drandm)
 !     param_dir = 'parameters'
 !     density = 0.01, gen_seed = 1   ! 'generate' only
+!     waves = 1             ! wavefronts per system (1 to 8; same results)
 !   /
 !
 ! 'generate' writes parameters/identity.inp, hp1.inp, hp2.inp,
@@ -34,11 +35,11 @@ module pdmd_driver_m
   implicit none
   character(len=16) :: mode = 'run', start = 'helix'
   character(len=256) :: param_dir = 'parameters'
-  integer :: nchains1 = 8, chnln1 = 16, nchains2 = 0, chnln2 = 0, canon = 1, list_capacity = 0
+  integer :: nchains1 = 8, chnln1 = 16, nchains2 = 0, chnln2 = 0, canon = 1, list_capacity = 0, waves = 1
   integer(c_int64_t) :: seed = 1, gen_seed = 1
   real(c_double) :: boxl = 0.0d0, density = 0.01d0
   namelist /pdmd_run/ mode, nchains1, chnln1, nchains2, chnln2, start, boxl, canon, seed, param_dir, density, &
-                      gen_seed, list_capacity
+                      gen_seed, list_capacity, waves
 
   real(c_double) :: tstar, setemp, sigma1
   integer(c_int64_t) :: ncoll
@@ -304,6 +305,7 @@ contains
     desc%bptnr = c_loc(bptnr)
     pd = trim(param_dir) // c_null_char
     call check(pdmd_ctx_create(pd, ctx), 'pdmd_ctx_create')
+    call check(pdmd_set_waves(ctx, int(waves, c_int32_t)), 'pdmd_set_waves')
     call check(pdmd_system_add(ctx, desc, sid), 'pdmd_system_add')
     write(6, '(a,a)') ' engine ', f_string(pdmd_version())
 

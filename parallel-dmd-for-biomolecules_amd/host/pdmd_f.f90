@@ -81,6 +81,11 @@ module pdmd_f
       integer(c_int32_t), intent(in) :: ids(*)
       integer(c_int64_t), intent(in) :: nevents(*)
     end function
+    integer(c_int) function pdmd_set_waves(ctx, waves_per_system) bind(C, name='pdmd_set_waves')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: waves_per_system
+    end function
     integer(c_int) function pdmd_advance_to_output(ctx, nsys, ids, max_nevents) bind(C, name='pdmd_advance_to_output')
       import :: c_int, c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: ctx

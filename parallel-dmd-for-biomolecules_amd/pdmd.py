@@ -111,7 +111,7 @@ EXPORTS = [
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
     "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
-    "pdmd_set_waves",
+    "pdmd_set_waves", "pdmd_autotune_waves",
 ]
 
 _lib = None
@@ -138,6 +138,7 @@ def lib():
         L.pdmd_wait.argtypes = [ctypes.c_void_p]
         L.pdmd_last_launch_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.pdmd_set_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.pdmd_autotune_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, ctypes.c_int64, _i32p, _f64p]
         L.pdmd_download.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
         L.pdmd_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]
         L.pdmd_log_enable.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
@@ -226,6 +227,16 @@ class Engine:
     def set_waves(self, waves):
         """Wavefronts per system for later advances (pdmd_set_waves)."""
         _check(lib().pdmd_set_waves(self._ctx, int(waves)))
+
+    def autotune_waves(self, ids, probe_events=2000):
+        """Pick the wavefronts per system by timing short advances of the
+        listed systems (pdmd_autotune_waves); returns (waves, {waves: rate})."""
+        a = np.ascontiguousarray(ids, np.int32)
+        w = ctypes.c_int32(0)
+        rates = np.zeros(4, np.float64)
+        _check(lib().pdmd_autotune_waves(self._ctx, len(a), _p(a, ctypes.c_int32), int(probe_events), ctypes.byref(w),
+                                         _p(rates, ctypes.c_double)))
+        return w.value, {k: float(r) for k, r in zip((1, 2, 4, 8), rates) if r > 0}
 
     def close(self):
         if self._ctx:

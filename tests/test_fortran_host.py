@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "parallel-dmd-for-biomolecules_amd", "host")
 DRIVER = os.path.join(HOST, "pdmd_driver")
 NML = "&pdmd_run\n mode='{mode}', nchains1=3, chnln1=8, nchains2=2, chnln2=6, density=0.01, gen_seed=5,\n" \
-      " canon=1, seed=3, start='helix' /\n"
+      " canon=1, seed=3, start='helix', waves={waves} /\n"
 
 
 @pytest.fixture(scope="module")
@@ -31,9 +31,9 @@ def driver():
     return DRIVER
 
 
-def _run(driver, work, mode, tstar=0.20, ncoll=1000):
+def _run(driver, work, mode, tstar=0.20, ncoll=1000, waves=1):
     with open(os.path.join(work, "pdmd.nml"), "w") as f:
-        f.write(NML.format(mode=mode))
+        f.write(NML.format(mode=mode, waves=waves))
     r = subprocess.run([driver], input=f"{tstar}\n{ncoll}\n", cwd=work, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
@@ -108,13 +108,14 @@ def _records(path):
 
 
 @pytest.mark.gpu
-def test_run_matches_oracle(driver, tmp_path, param_dir):
+@pytest.mark.parametrize("waves", [1, 4])
+def test_run_matches_oracle(driver, tmp_path, param_dir, waves):
     from oracle import Oracle
 
     work = _workdir(tmp_path)
     _run(driver, work, "generate")
     ncoll = 3000
-    out = _run(driver, work, "run", ncoll=ncoll)
+    out = _run(driver, work, "run", ncoll=ncoll, waves=waves)
     assert "the final total energy" in out
     ref = _python_system()
     N, B, sv = _read_helix(work)
