@@ -30,6 +30,8 @@ typedef struct pdmd_ctx pdmd_ctx;
  * nop1/nop2 blocks (NULL = every chain its own species).  seed keys the
  * project RNG that replaces IFPORT drandm; canon != 0 enables Andersen
  * ghost collisions (-Dcanon). */
+#define PDMD_NO_HBS 1 /* no N-C hydrogen-bond wells: the reference's code 15 pairs stay hard cores */
+
 typedef struct {
     int32_t nchains;
     const int32_t* chain_len;
@@ -41,6 +43,8 @@ typedef struct {
     uint64_t seed;
     int32_t canon;
     int32_t list_capacity; /* per-bead up/down list slots, at most 64; 0 = default (64) */
+    int32_t flags;         /* PDMD_NO_HBS: build flag -Dno_hbs (make_code.f:114-170,192-208,348-365) */
+    int32_t pad;
     const double* sv;
     const int32_t* bptnr;
 } pdmd_system_desc;
@@ -164,6 +168,25 @@ int pdmd_pair_codes(const pdmd_system_desc* desc, int32_t n, const int32_t* a, c
 
 /* Observables of the output pseudo-event (energy.f, radgyr.f, end2end.f). */
 int pdmd_observables(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
+
+typedef struct {
+    int64_t overlaps;    /* checkover.f:21-131: pairs with ev_code 1 or >= 15 closer than sigma */
+    int64_t bonds_long;  /* checkover.f: bonded pairs (codes 4-12) longer than their window */
+    int64_t bonds_short; /* ... and shorter */
+    int32_t first_i, first_j; /* 1-based first offending pair in (i, j) order; 0 if none */
+    int64_t boundbad;    /* check_nc_int.f:25-354: H-bonded N-C pairs whose angle test fails */
+    int64_t unboundbad;  /* ... N-C pairs inside the well, angle test passing, not bonded */
+    int64_t ss_counted;  /* ... n_ss: non-terminal such pairs with their shoulder partner set */
+    int64_t ss_total;    /* ... m_ss: listed code-15 pairs with extra_repuls(i,4) = j; the
+                            reference stops when it differs from n_ss */
+} pdmd_check_t;
+/* Debug checks of one system on the device: checkover.f (all pairs) and
+ * check_nc_int.f (listed code-15 pairs), at the system's current tfalse, on
+ * the resident state or, when sv is not NULL, on the caller's sv(6,N) with
+ * the system's topology, identities and codes (main.F90:347,425,1194-1204,
+ * 1322 call them at the start, at output events under -Ddebugging, at the
+ * end). */
+int pdmd_check(pdmd_ctx* ctx, int32_t sys_id, const double* sv, pdmd_check_t* out);
 
 /* Seeded synthetic input (no reference counterpart: the shipped generator
  * genconfig/gen_config_random-SQZ.f90 is template-based and IFPORT-seeded):

@@ -69,6 +69,7 @@ def lib():
         L.orc_predict.restype = ctypes.c_double
         L.orc_predict.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p]
         L.orc_checkover.argtypes = [ctypes.c_void_p]
+        L.orc_check.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_table.restype = ctypes.c_double
         L.orc_table.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_bond.restype = ctypes.c_double
@@ -159,6 +160,15 @@ class Oracle:
 
     def checkover(self):
         return lib().orc_checkover(self.h)
+
+    def check(self, sv=None):
+        """checkover.f split by kind and check_nc_int.f: dict of counts."""
+        out = np.zeros(9, np.int64)
+        keep = None if sv is None else np.ascontiguousarray(sv, np.float64)
+        lib().orc_check(self.h, None if keep is None else keep.ctypes.data, out.ctypes.data)
+        k = ("overlaps", "bonds_long", "bonds_short", "first_i", "first_j", "boundbad", "unboundbad", "ss_counted",
+             "ss_total")
+        return {n: int(v) for n, v in zip(k, out)}
 
     def table(self, which, k, kk=0):
         return lib().orc_table(self.h, which, k, kk)

@@ -1754,6 +1754,85 @@ int32_t orc_checkover(const orc_sys* S) {
     return bad;
 }
 
+/* checkover.f:21-131 with the violations split by kind, then
+ * check_nc_int.f:25-354 over each bead's up list (code-15 partners):
+ * out = {overlaps, bonds too long, bonds too short, first i, first j (1-based,
+ * 0 if none), boundbad, unboundbad, n_ss, m_ss}.  sv (the reference layout)
+ * replaces the state's positions and velocities when not NULL. */
+void orc_check(orc_sys* S, const double* sv, int64_t out[9]) {
+    double* keep = S->sv;
+    if (sv) S->sv = (double*)sv;
+    const params_t* P = &S->P;
+    int N = S->N;
+    for (int k = 0; k < 9; ++k) out[k] = 0;
+    for (int i = 1; i <= N - 1; ++i)
+        for (int j = i + 1; j <= N; ++j) {
+            int evcode = EV(S, j, i), bad = 0;
+            pairgeo g;
+            if (evcode == 1 || evcode >= 15) {
+                double pe = P->ev_param[1][evcode];
+                double sigsq = P->sigma_sq[S->identity[i]][S->identity[j]] * (pe * pe);
+                if (evcode >= 22 && evcode <= 26) {
+                    int k = S->identity[i] > S->identity[j] ? S->identity[i] : S->identity[j];
+                    double q = P->sqz610[evcode - 21][k];
+                    sigsq = sigsq * (q * q);
+                }
+                geo(S, i, j, S->tfalse, &g);
+                double r2 = g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
+                r2 = r2 * 1.0000000001;
+                if (r2 <= sigsq) {
+                    out[0]++;
+                    bad = 1;
+                }
+            } else if (evcode >= 4 && evcode <= 12) {
+                double blmin, blmax;
+                bond_window(S, i, evcode, &blmin, &blmax);
+                blmin = blmin * blmin;
+                blmax = blmax * blmax;
+                geo(S, i, j, S->tfalse, &g);
+                double r2 = g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
+                if (r2 * 0.9999999999 > blmax) {
+                    out[1]++;
+                    bad = 1;
+                }
+                if (r2 * 1.0000000001 < blmin) {
+                    out[2]++;
+                    bad = 1;
+                }
+            }
+            if (bad && out[3] == 0) {
+                out[3] = i;
+                out[4] = j;
+            }
+        }
+    for (int i = 1; i <= N; ++i) {
+        int ks = (i - 1) * MAXNBS;
+        for (int k = 0; k < S->na_npt[i]; ++k) {
+            int j = S->nb[ks + k];
+            if (EV(S, j, i) != 15) continue;
+            int term = terminal(S, i) || terminal(S, j);
+            double rating;
+            if (S->bptnr[i] == j) {
+                rating = term ? 10.0 : (S->identity[i] < S->identity[j] ? repuls_check(S, i, j) : repuls_check(S, j, i));
+                if (rating > 10.0) out[5]++;
+                if (!term && XR(S, i, 4) == j) out[7]++;
+            } else {
+                pairgeo g;
+                geo(S, i, j, S->tfalse, &g);
+                double rijsq = g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
+                double diff = rijsq - P->welldia_sq[S->identity[i]][S->identity[j]];
+                if (diff < 0.0) {
+                    rating = term ? 11.0 : (S->identity[i] < S->identity[j] ? repuls_check(S, i, j) : repuls_check(S, j, i));
+                    if (rating <= 10.0) out[6]++;
+                    if (!term && XR(S, i, 4) == j) out[7]++;
+                }
+            }
+            if (XR(S, i, 4) == j) out[8]++;
+        }
+    }
+    S->sv = keep;
+}
+
 /* ------------------------------------------------------------------ */
 /* Construction: main.F90:126-424 with -Drunr -Dcanon                   */
 /* ------------------------------------------------------------------ */

@@ -85,6 +85,8 @@ int32_t orc_cell_of(const orc_sys* s, int32_t i);
 double orc_predict(const orc_sys* s, int32_t i, int32_t j, int32_t* type);
 /* Overlap check restating checkover.f; returns number of violations. */
 int32_t orc_checkover(const orc_sys* s);
+/* checkover.f split by kind plus check_nc_int.f (see pdmd_oracle.c) */
+void orc_check(orc_sys* s, const double* sv, int64_t out[9]);
 /* Table access for tests: which = 0 sigma_sq, 1 welldia_sq, 2 ep_sqrt,
  * 3 shlddia_sq, 4 sigma_2b, 5 bmass(k), 6 ev_param(k=row,kk=col),
  * 7 sqz610(k,kk), 8 rlsq(k). */

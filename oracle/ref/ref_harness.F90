@@ -36,7 +36,9 @@
 	integer, allocatable :: coltypes(:)
 	real*8 tstar_in, tij, tmin, ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg
 	integer hb_alpha, hb_ii, hb_ij
-	logical direct
+	logical direct, over
+	integer boundbad, unboundbad
+	real*8, allocatable :: svk(:,:)
 	real*8 ev_rec(2)
 	integer ev_int(3)
 
@@ -290,6 +292,31 @@
 	call end_to_end(e2e_avg)
 	write(22) ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg, hb_alpha, hb_ii, hb_ij
 	close(22)
+
+	! checkover.f and check_nc_int.f on the final state, then checkover.f on
+	! two planted states (bead 5 on top of bead noptotal-3; bead 10 moved by
+	! 0.05 box lengths along x); their messages go to stdout between the
+	! CHECK and CHECKEND lines
+	write(6,'(a)') 'CHECK final'
+	call checkover(over)
+	boundbad = 0
+	unboundbad = 0
+	call check_nc_int(boundbad, unboundbad)
+	write(6,'(a,l2,2i8)') 'CHECKEND ', over, boundbad, unboundbad
+	allocate(svk(6, noptotal))
+	svk(:, 1:noptotal) = sv(:, 1:noptotal)
+	sv(1,5) = sv(1,noptotal-3) + 1.d-3
+	sv(2,5) = sv(2,noptotal-3) + 1.d-3
+	sv(3,5) = sv(3,noptotal-3) + 1.d-3
+	write(6,'(a)') 'CHECK planted1'
+	call checkover(over)
+	write(6,'(a,l2)') 'CHECKEND ', over
+	sv(:, 1:noptotal) = svk(:, 1:noptotal)
+	sv(1,10) = sv(1,10) + 5.d-2
+	write(6,'(a)') 'CHECK planted2'
+	call checkover(over)
+	write(6,'(a,l2)') 'CHECKEND ', over
+	sv(:, 1:noptotal) = svk(:, 1:noptotal)
 	write(6,'(a,i8,a,f10.4)') 'harness: pair events committed ', npair, ' avg nbors ', dble(nbrsum)/dble(noptotal)
 	end program ref_harness
 
@@ -320,3 +347,6 @@
 #include "sqwel.f"
 #include "radgyr.f"
 #include "end2end.f"
+#include "repuls_check.f"
+#include "checkover.f"
+#include "check_nc_int.f"

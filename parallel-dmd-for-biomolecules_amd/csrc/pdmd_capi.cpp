@@ -160,6 +160,7 @@ int validate_desc(const pdmd_system_desc* d) {
     for (int c = 0; c < d->nchains; ++c)
         if (d->chain_len[c] < 4) return fail("pdmd_system_desc: chains need at least 4 residues");
     if (!(d->boxl > 0) || !(d->tstar > 0)) return fail("pdmd_system_desc: boxl and tstar must be positive");
+    if (d->flags & ~PDMD_NO_HBS) return fail("pdmd_system_desc: unknown flags");
     return 0;
 }
 
@@ -252,6 +253,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     for (int q = 0; q < nres; ++q)
         if (rt[q] < 9 || rt[q] > 28) return fail("pdmd_system_add: restype must be 9..28");
     H.topo.build(clen, spec, rt, hp);
+    H.topo.no_hbs = (d->flags & PDMD_NO_HBS) != 0;
     const Topology& t = H.topo;
     const int N = t.N;
     H.N = N;
@@ -368,7 +370,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
         const int c = t.chain[k], L = t.chain_len[c], kind = (t.loc[k] - 1) / L, r = (t.loc[k] - 1) % L + 1;
         const int q = t.res[k];
         const int pro = rt[q] == 17 ? 1 : 0;
-        bead[k].topo = make_topo(c, kind, r, L, hp[q] ? 1 : 0, pro);
+        bead[k].topo = make_topo(c, kind, r, L, hp[q] ? 1 : 0, pro, t.no_hbs ? 1 : 0);
         bead[k].dyn = make_dyn(ident[k], bptnr[k]);
         if (kind <= 2) {
             const int iii = rt[q] - 8;
@@ -784,6 +786,48 @@ int pdmd_predict_pairs(pdmd_ctx* ctx, int32_t id, int32_t n, const int32_t* pi, 
     return 0;
 }
 
+int pdmd_check(pdmd_ctx* ctx, int32_t id, const double* sv, pdmd_check_t* out) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H || !out) return fail("pdmd_check: bad arguments");
+    const int N = H->N;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    DevBuf<Rec> alt;
+    if (sv) {
+        // the caller's sv(6,N) in place of the resident positions/velocities
+        std::vector<Rec> b(N);
+        HIPCHK(hipMemcpy(b.data(), H->rec.p, sizeof(Rec) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) {
+            const double* q = sv + 6 * (size_t)k;
+            b[k].x = q[0];
+            b[k].y = q[1];
+            b[k].z = q[2];
+            b[k].vx = q[3];
+            b[k].vy = q[4];
+            b[k].vz = q[5];
+        }
+        HIPCHK(alt.alloc(N));
+        HIPCHK(hipMemcpy(alt.p, b.data(), sizeof(Rec) * N, hipMemcpyHostToDevice));
+    }
+    DevBuf<unsigned long long> cnt;
+    HIPCHK(cnt.alloc(8));
+    unsigned long long init[8] = {0, 0, 0, ~0ull, 0, 0, 0, 0};
+    HIPCHK(hipMemcpy(cnt.p, init, sizeof(init), hipMemcpyHostToDevice));
+    HIPCHK(engine_check(ctx->d_sys.p, id - 1, N, sv ? alt.p : nullptr, cnt.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    unsigned long long r[8];
+    HIPCHK(hipMemcpy(r, cnt.p, sizeof(r), hipMemcpyDeviceToHost));
+    out->overlaps = (int64_t)r[0];
+    out->bonds_long = (int64_t)r[1];
+    out->bonds_short = (int64_t)r[2];
+    out->first_i = r[3] == ~0ull ? 0 : (int32_t)(r[3] / (unsigned long long)N) + 1;
+    out->first_j = r[3] == ~0ull ? 0 : (int32_t)(r[3] % (unsigned long long)N) + 1;
+    out->boundbad = (int64_t)r[4];
+    out->unboundbad = (int64_t)r[5];
+    out->ss_counted = (int64_t)r[6];
+    out->ss_total = (int64_t)r[7];
+    return 0;
+}
+
 int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
     HostSys* H = get_sys(ctx, id);
     if (!H || !o) return -1;
@@ -915,6 +959,7 @@ int pdmd_pair_codes(const pdmd_system_desc* d, int32_t n, const int32_t* a, cons
     std::vector<int> rt(d->restype, d->restype + nres), hp(d->hp, d->hp + nres);
     Topology t;
     t.build(clen, spec, rt, hp);
+    t.no_hbs = (d->flags & PDMD_NO_HBS) != 0;
     for (int k = 0; k < n; ++k) {
         if (a[k] < 1 || a[k] > t.N || b[k] < 1 || b[k] > t.N) return fail("pdmd_pair_codes: bead index out of range");
         code[k] = a[k] == b[k] ? 1 : static_code(t, a[k] - 1, b[k] - 1);

@@ -19,7 +19,8 @@
 namespace pdmd {
 
 // topo word: chain (13 bits) | kind (2: 0 Calpha, 1 N, 2 C, 3 R) | residue-1
-// (7) | chain length-1 (7) | hp (1) | proline residue (1)
+// (7) | chain length-1 (7) | hp (1) | proline residue (1) | system built
+// with -Dno_hbs (1, the sign bit)
 // dyn word:  identity (5 bits) | (H-bond partner + 1) << 5
 struct __attribute__((aligned(16))) Rec {
     double x, y, z, vx, vy, vz, bm;
@@ -38,8 +39,11 @@ __host__ __device__ inline int topo_res(int t) { return ((t >> 15) & 127) + 1; }
 __host__ __device__ inline int topo_len(int t) { return ((t >> 22) & 127) + 1; }
 __host__ __device__ inline int topo_hp(int t) { return (t >> 29) & 1; }
 __host__ __device__ inline int topo_pro(int t) { return (t >> 30) & 1; }
-__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro) {
-    return (chain & 0x1FFF) | (kind << 13) | ((r - 1) << 15) | ((L - 1) << 22) | (hp << 29) | (pro << 30);
+__host__ __device__ inline int topo_nohbs(int t) { return t < 0 ? 1 : 0; }
+__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro, int nohbs) {
+    return (int)((unsigned)((chain & 0x1FFF) | (kind << 13) | ((r - 1) << 15) | ((L - 1) << 22) | (hp << 29) |
+                            (pro << 30)) |
+                 ((unsigned)nohbs << 31));
 }
 __host__ __device__ inline int dyn_ident(int d) { return d & 31; }
 __host__ __device__ inline int dyn_bptnr(int d) { return (d >> 5) - 1; }

@@ -25,6 +25,10 @@ hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n,
 constexpr int kStopAtOutput = 1;  // engine_run flag: stop after the next output pseudo-event
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
                       size_t lds_limit, hipStream_t s, long long budget_ticks = 0, int flags = 0, int waves = 1);
+// Debug checks (checkover.f, check_nc_int.f) of system id's n beads: out[8]
+// counters (zeroed by the caller, out[3] preset to all ones); alt replaces
+// the resident state records when not null.
+hipError_t engine_check(const SysDev* d_sys, int id, int n, const Rec* alt, unsigned long long* out, hipStream_t s);
 // Test hook: collision times of (i, j) pairs (1-based) of system id.
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s);

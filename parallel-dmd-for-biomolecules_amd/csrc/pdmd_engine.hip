@@ -192,7 +192,7 @@ __device__ double dlog(double x) {
 // ------------------------------------------------------------------------
 // pair codes: closed form of make_code.f from two topo words, overrides
 // ------------------------------------------------------------------------
-__device__ inline int static_t(int ta, int tb) {
+__device__ inline int static_t0(int ta, int tb) {
     int ka = topo_kind(ta), ra = topo_res(ta), kb = topo_kind(tb), rb = topo_res(tb);
     if (topo_chain(ta) == topo_chain(tb)) {
         if (ka > kb || (ka == kb && ra > rb)) {
@@ -226,6 +226,12 @@ __device__ inline int static_t(int ta, int tb) {
     if (ka == 1 && kb == 2) return (topo_pro(ta) && topo_chain(ta) < topo_chain(tb)) ? 1 : 15;
     if (ka == 2 && kb == 1) return (topo_pro(tb) && topo_chain(tb) < topo_chain(ta)) ? 1 : 15;
     return 1;
+}
+// under -Dno_hbs make_code.f skips every code-15 assignment (:114-170,
+// :192-208, :348-365): those pairs keep the hard-core code 1
+__device__ inline int static_t(int ta, int tb) {
+    const int r = static_t0(ta, tb);
+    return (r == 15 && topo_nohbs(ta)) ? 1 : r;
 }
 // ev_code(row, col) with row's override slots o
 __device__ inline int dyn_code(int4 o, int col, int stat) {
@@ -552,8 +558,11 @@ __device__ inline int calc_bin(const Sc& c, double tim, bool uncond, int& status
     return j < c.nbin ? -1 : j;
 }
 
+// flags: kPairReadsSp = the outcome resolution read the shoulder set or an
+// H-bond partner; kPairWroteSp = the event rewrote them
+constexpr int kPairReadsSp = 1, kPairWroteSp = 2;
 struct Pair {  // the event's colliding pair (j < 0 for a ghost event)
-    int i, j;
+    int i, j, flags, pad;
     Rec ri, rj;
     double2 bwi, bwj;
     int4 xi, xj;
@@ -1471,7 +1480,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         if (dent >= 0) mark_or(S.mark + dent, lbit);
         if (lane == 0) mark_or(S.mark + i, sbit | lbit);
         if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit);
-        if (bk == 0 && sp >= 0) mark_or(S.mark + sp, sbit);
+        const bool sp_w = (P.flags & kPairWroteSp) != 0, sp_r = (P.flags & kPairReadsSp) != 0;
+        if (bk == 0 && sp_w && sp >= 0) mark_or(S.mark + sp, sbit);
         bar_to(nb, 1);
         PH_END(S, kPhBMark, tph);
         if (bk >= 1) {
@@ -1487,7 +1497,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             if (dent >= 0 && (mark_get(S.mark + dent) & (lowS | lowL))) bad = true;
             if (lane == 0 && (mark_get(S.mark + i) & (lowS | lowL))) bad = true;
             if (lane == 1 && j >= 0 && (mark_get(S.mark + j) & (lowS | lowL))) bad = true;
-            if (sp >= 0 && (mark_get(S.mark + sp) & lowS)) bad = true;
+            if (sp_r && sp >= 0 && (mark_get(S.mark + sp) & lowS)) bad = true;
             // a down bead whose full recompute is not in the batch would
             // read rows that were not checked
             const bool need_full = first && !is_i &&
@@ -1501,7 +1511,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         if (dent >= 0) mark_clear(S.mark + dent);
         if (lane == 0) mark_clear(S.mark + i);
         if (lane == 1 && j >= 0) mark_clear(S.mark + j);
-        if (bk == 0 && sp >= 0) mark_clear(S.mark + sp);
+        if (bk == 0 && sp_w && sp >= 0) mark_clear(S.mark + sp);
         if (bk >= B.cut) {
             bar_to(nb, 3);
             return;
@@ -2048,6 +2058,8 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     Pair P;
     P.i = i;
     P.j = j;
+    P.flags = 0;
+    P.pad = 0;
     P.ri = gld(S.rec + (i));
     P.rj = gld(S.rec + (j));
     P.bwi = gld(S.bw + (i));
@@ -2060,6 +2072,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     const int idi = dyn_ident(P.ri.dyn), idj = dyn_ident(P.rj.dyn);
     int status = 0;
     bool xpulse = false, fence = false;
+    if (ct == 7 || ct == 10 || ct == 12) P.flags |= kPairReadsSp;
     if (ct == 7) {
         if (P.xi.w < 0 && P.xj.w < 0) {
             if (ok) {
@@ -2134,6 +2147,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
             }
         }
     } else if (ct == 24 || ct == 25) {
+        P.flags |= kPairReadsSp;
         const bool xi_ = evcode < 45;
         const int hbp = xi_ ? P.xi.w : P.xj.w;
         const int x = xi_ ? i : j;
@@ -2192,6 +2206,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
         gst(S.rec + (j), P.rj);
     }
     if (lane == 0 && bk >= 0) L.batch()->ty[bk] = ct;
+    if (fence) P.flags |= kPairWroteSp;
     if (fence) {
         // shoulder tables changed: make them visible before the predictions
         wave_sync_mem();
@@ -2227,6 +2242,8 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
     Pair P;
     P.i = i;
     P.j = -1;
+    P.flags = 0;
+    P.pad = 0;
     P.ri = gld(S.rec + (i));
     P.bwi = gld(S.bw + (i));
     P.xi = gld(S.extra + (i));
@@ -3115,6 +3132,106 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
     }
 }
 
+// ------------------------------------------------------------------------
+// debug checks of one system's state (one lane per bead i)
+// ------------------------------------------------------------------------
+// repuls_check.f:16-81 on a state array (n = the N bead, cb = the C bead);
+// 11 when a shoulder bead falls outside the system
+__device__ double check_rating(const DevTables& T, const Rec* rec, int N, int n, int cb, double tf) {
+    const Rec rn = rec[n], rc = rec[cb];
+    int ncim1, ncai, ncaj, nnjp1;
+    shoulder_idx(n, topo_len(rn.topo), cb, topo_len(rc.topo), ncim1, ncai, ncaj, nnjp1);
+    if (ncim1 < 0 || ncim1 >= N || ncai < 0 || ncai >= N || ncaj < 0 || ncaj >= N || nnjp1 < 0 || nnjp1 >= N)
+        return 11.0;
+    const double d1 = dist_rec(rn, rec[ncaj], tf), d2 = dist_rec(rn, rec[nnjp1], tf);
+    const double d3 = dist_rec(rc, rec[ncai], tf), d4 = dist_rec(rc, rec[ncim1], tf);
+    if (d1 > T.shder[1] && d2 > T.shder[2] && d3 > T.shder[3] && d4 > T.shder[4]) return 10.0;
+    return 11.0;
+}
+// out[0..2]: checkover.f:21-131 over all pairs (i < j) at tfalse: cores
+// closer than sigma (codes 1 and >= 15, squeezed for 22-26), bonds longer /
+// shorter than their window (4-12); out[3]: least i*N + j among them;
+// out[4..7]: check_nc_int.f:25-354 over i's listed code-15 partners:
+// boundbad, unboundbad, n_ss, m_ss.  alt: positions and velocities to check
+// in place of the resident ones (topology and codes stay the system's).
+__global__ void k_check(const SysDev* sys, int id, const Rec* alt, unsigned long long* out) {
+    const SysDev& S = sys[id];
+    const DevTables& T = *S.tab;
+    const int N = S.N, cap = S.cap;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= N) return;
+    const Rec* rec = alt ? alt : S.rec;
+    const double tf = S.st->tfalse;
+    const Rec ri = rec[i];
+    const int idi = dyn_ident(ri.dyn);
+    unsigned long long over = 0, lng = 0, sht = 0, first = ~0ull;
+    for (int j = i + 1; j < N; ++j) {
+        const Rec rj = rec[j];
+        const int idj = dyn_ident(rj.dyn);
+        const int code = dyn_code(S.ovr[j], i, static_t(rj.topo, ri.topo));  // ev_code(j, i)
+        if (code == 1 || code >= 15) {
+            const double pe = T.ev_param[1][code];
+            double sigsq = T.sigma_sq[idi][idj] * (pe * pe);
+            if (code >= 22 && code <= 26) {
+                const double q = T.sqz610[code - 21][idi > idj ? idi : idj];
+                sigsq = sigsq * (q * q);
+            }
+            Geo g;
+            geo(ri, rj, tf, g);
+            const double r2 = g.r2 * 1.0000000001;
+            if (r2 <= sigsq) {
+                ++over;
+                first = min(first, (unsigned long long)i * (unsigned long long)N + (unsigned long long)j);
+            }
+        } else if (code >= 4 && code <= 12) {
+            double blmin, blmax;
+            bond_window(T, code, S.bw[i], blmin, blmax);
+            blmin = blmin * blmin;
+            blmax = blmax * blmax;
+            Geo g;
+            geo(ri, rj, tf, g);
+            const bool lo = g.r2 * 1.0000000001 < blmin, hi = g.r2 * 0.9999999999 > blmax;
+            if (hi) ++lng;
+            if (lo) ++sht;
+            if (lo || hi) first = min(first, (unsigned long long)i * (unsigned long long)N + (unsigned long long)j);
+        }
+    }
+    unsigned long long bb = 0, ub = 0, nss = 0, mss = 0;
+    const int nu = S.nup[i];
+    const int4 xi = S.extra[i];
+    for (int k = 0; k < nu && k < cap; ++k) {
+        const int ent = S.nb_up[(size_t)i * cap + k];
+        const int j = ent & 0xFFFFFF;
+        if (j < 0 || j >= N) continue;
+        const Rec rj = rec[j];
+        const int idj = dyn_ident(rj.dyn);
+        if (dyn_code(S.ovr[j], i, ent >> 24) != 15) continue;
+        const bool term = terminal_t(ri.topo) || terminal_t(rj.topo);
+        if (dyn_bptnr(ri.dyn) == j) {
+            const double rating = term ? 10.0 : (idi < idj ? check_rating(T, rec, N, i, j, tf) : check_rating(T, rec, N, j, i, tf));
+            if (rating > 10.0) ++bb;
+            if (!term && xi.w == j) ++nss;
+        } else {
+            Geo g;
+            geo(ri, rj, tf, g);
+            if (g.r2 - T.welldia_sq[idi][idj] < 0.0) {
+                const double rating = term ? 11.0 : (idi < idj ? check_rating(T, rec, N, i, j, tf) : check_rating(T, rec, N, j, i, tf));
+                if (rating <= 10.0) ++ub;
+                if (!term && xi.w == j) ++nss;
+            }
+        }
+        if (xi.w == j) ++mss;
+    }
+    if (over) atomicAdd(out + 0, over);
+    if (lng) atomicAdd(out + 1, lng);
+    if (sht) atomicAdd(out + 2, sht);
+    if (first != ~0ull) atomicMin(out + 3, first);
+    if (bb) atomicAdd(out + 4, bb);
+    if (ub) atomicAdd(out + 5, ub);
+    if (nss) atomicAdd(out + 6, nss);
+    if (mss) atomicAdd(out + 7, mss);
+}
+
 // test hook: predicted collision time of listed pairs (1-based in, code
 // from ev_code(j,i) as events.f/eventredo_up.f read it)
 __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const int* pj, double* tij, int* type) {
@@ -3183,6 +3300,10 @@ int engine_max_waves(int max_n, int maxw, size_t lds_limit) {
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_predict, dim3((n + 255) / 256), dim3(256), 0, s, d_sys, id, n, pi, pj, tij, type);
+    return hipGetLastError();
+}
+hipError_t engine_check(const SysDev* d_sys, int id, int n, const Rec* alt, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_check, dim3((n + 255) / 256), dim3(256), 0, s, d_sys, id, alt, out);
     return hipGetLastError();
 }
 hipError_t engine_prepare(size_t lds_limit) {

@@ -251,7 +251,7 @@ int static_code(const Topology& t, int a, int b) {
                 if (d == -1) return 6;
                 if (d == 0) return 8;
                 if (d == -2) return 19;
-                return (d >= -3 && d <= 3) ? 1 : 15;
+                return (d >= -3 && d <= 3) || t.no_hbs ? 1 : 15;
             case 1 * 4 + 3: return d == 0 ? 11 : (d == -1 ? 23 : 1);                    // N-R
             case 2 * 4 + 2: return (d == 1) ? 21 : 1;                                    // C-C
             case 2 * 4 + 3: return d == 0 ? 12 : (d == 1 ? 22 : (d == 2 ? 26 : 1));     // C-R
@@ -263,6 +263,7 @@ int static_code(const Topology& t, int a, int b) {
     // different chains
     if (ka == 3 && kb == 3) return (t.hp[t.res[a]] == 1 && t.hp[t.res[b]] == 1) ? 16 : 1;
     if ((ka == 1 && kb == 2) || (ka == 2 && kb == 1)) {
+        if (t.no_hbs) return 1;  // make_code.f:114-170 skipped under -Dno_hbs
         // proline NH rule (make_code.f:139-161): kept only where the Pro side
         // chain precedes the CO bead in index order, i.e. the N's chain first.
         const int nb = ka == 1 ? a : b, cb = ka == 1 ? b : a;

@@ -23,6 +23,8 @@
 !     param_dir = 'parameters'
 !     density = 0.01, gen_seed = 1   ! 'generate' only
 !     waves = 1             ! wavefronts per system (1 to 8; same results)
+!     no_hbs = 0            ! 1: build flag -Dno_hbs (no N-C H-bond wells)
+!     debugging = 0         ! 1: -Ddebugging checks at every output event
 !   /
 !
 ! 'generate' writes parameters/identity.inp, hp1.inp, hp2.inp,
@@ -35,11 +37,11 @@ module pdmd_driver_m
   implicit none
   character(len=16) :: mode = 'run', start = 'helix'
   character(len=256) :: param_dir = 'parameters'
-  integer :: nchains1 = 8, chnln1 = 16, nchains2 = 0, chnln2 = 0, canon = 1, list_capacity = 0, waves = 1
+  integer :: nchains1 = 8, chnln1 = 16, nchains2 = 0, chnln2 = 0, canon = 1, list_capacity = 0, waves = 1, no_hbs = 0, debugging = 0
   integer(c_int64_t) :: seed = 1, gen_seed = 1
   real(c_double) :: boxl = 0.0d0, density = 0.01d0
   namelist /pdmd_run/ mode, nchains1, chnln1, nchains2, chnln2, start, boxl, canon, seed, param_dir, density, &
-                      gen_seed, list_capacity, waves
+                      gen_seed, list_capacity, waves, no_hbs, debugging
 
   real(c_double) :: tstar, setemp, sigma1
   integer(c_int64_t) :: ncoll
@@ -164,6 +166,8 @@ contains
     type(pdmd_system_desc) :: desc
     type(pdmd_stats_t) :: st
     type(pdmd_obs_t) :: ob
+    type(pdmd_check_t) :: chk
+    integer(c_int64_t) :: boundbad, unboundbad
     type(c_ptr) :: ctx
     integer(c_int32_t) :: sid
     integer(c_int64_t) :: coll, coll_in, before, left
@@ -301,6 +305,7 @@ contains
     desc%seed = seed
     desc%canon = int(canon, c_int32_t)
     desc%list_capacity = int(list_capacity, c_int32_t)
+    desc%flags = merge(1_c_int32_t, 0_c_int32_t, no_hbs /= 0)
     desc%sv = c_loc(sv)
     desc%bptnr = c_loc(bptnr)
     pd = trim(param_dir) // c_null_char
@@ -308,6 +313,16 @@ contains
     call check(pdmd_set_waves(ctx, int(waves, c_int32_t)), 'pdmd_set_waves')
     call check(pdmd_system_add(ctx, desc, sid), 'pdmd_system_add')
     write(6, '(a,a)') ' engine ', f_string(pdmd_version())
+    ! main.F90:345-350 and 425: the starting configuration must be free of
+    ! overlaps and bond violations
+    call check(pdmd_check(ctx, sid, c_null_ptr, chk), 'pdmd_check')
+    if (chk%overlaps + chk%bonds_long + chk%bonds_short > 0) then
+      write(6, *) 'found overlap/underlap in initial configuration'
+      write(6, *) 'particles ', chk%first_i, ' and ', chk%first_j
+      call exit(-1)
+    end if
+    boundbad = chk%boundbad
+    unboundbad = chk%unboundbad
 
     open(newunit=ue, file='results/run' // digits // '.energy', status='new')
     call check(pdmd_stats(ctx, sid, st), 'pdmd_stats')
@@ -325,11 +340,16 @@ contains
       if (st%coll - before < left) then
         call observables_line(coll)
         call write_config(coll)
+        if (debugging /= 0) call debug_checks(coll)
       end if
     end do
     call observables_line(coll - 1)  ! main.F90:1303
     call write_config(coll)
     close(ue)
+    ! main.F90:1320-1322: the end configuration is checked too
+    call check(pdmd_check(ctx, sid, c_null_ptr, chk), 'pdmd_check')
+    if (chk%overlaps + chk%bonds_long + chk%bonds_short > 0) &
+      write(6, *) 'overlap/underlap in the end configuration: particles ', chk%first_i, ' and ', chk%first_j
     call check(pdmd_observables(ctx, sid, ob), 'pdmd_observables')
     write(6, *) ' '
     write(6, *) 'at coll', coll
@@ -344,6 +364,27 @@ contains
     call check(pdmd_ctx_destroy(ctx), 'pdmd_ctx_destroy')
 
   contains
+
+    ! main.F90:1194-1204 (-Ddebugging): overlaps stop the run; changes of
+    ! the H-bond consistency counts are reported
+    subroutine debug_checks(at)
+      integer(c_int64_t), intent(in) :: at
+      call check(pdmd_check(ctx, sid, c_null_ptr, chk), 'pdmd_check')
+      if (chk%overlaps + chk%bonds_long + chk%bonds_short > 0) then
+        write(6, *) 'found overlap/underlap at coll', at
+        write(6, *) 'particles ', chk%first_i, ' and ', chk%first_j
+        call exit(-1)
+      end if
+      if (chk%boundbad /= boundbad .or. chk%unboundbad /= unboundbad) then
+        write(6, *) 'boundbad and unboundbad', chk%boundbad, chk%unboundbad
+        boundbad = chk%boundbad
+        unboundbad = chk%unboundbad
+      end if
+      if (chk%ss_counted /= chk%ss_total) then
+        write(6, *) 'wrong ss at coll', at, chk%ss_counted, chk%ss_total
+        call exit(-1)
+      end if
+    end subroutine debug_checks
 
     subroutine codes_out()
       integer(c_int32_t), allocatable :: a(:), b(:), cd(:)

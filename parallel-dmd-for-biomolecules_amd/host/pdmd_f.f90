@@ -18,6 +18,8 @@ module pdmd_f
     integer(c_int64_t) :: seed
     integer(c_int32_t) :: canon
     integer(c_int32_t) :: list_capacity
+    integer(c_int32_t) :: flags = 0
+    integer(c_int32_t) :: pad = 0
     type(c_ptr) :: sv = c_null_ptr
     type(c_ptr) :: bptnr = c_null_ptr
   end type pdmd_system_desc
@@ -36,6 +38,12 @@ module pdmd_f
     integer(c_int32_t) :: hb_alpha, hb_ii, hb_ij, pad
     real(c_double) :: ehh_ii, ehh_ij, rg_avg, e2e_avg
   end type pdmd_obs_t
+
+  type, bind(C) :: pdmd_check_t
+    integer(c_int64_t) :: overlaps, bonds_long, bonds_short
+    integer(c_int32_t) :: first_i, first_j
+    integer(c_int64_t) :: boundbad, unboundbad, ss_counted, ss_total
+  end type pdmd_check_t
 
   type, bind(C) :: pdmd_event_t
     integer(c_int64_t) :: coll
@@ -100,6 +108,13 @@ module pdmd_f
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: sys_id
       type(c_ptr), value :: sv, identity, bptnr, extra_repuls, tim, nptnr, coltype
+    end function
+    integer(c_int) function pdmd_check(ctx, sys_id, sv, out) bind(C, name='pdmd_check')
+      import :: c_int, c_ptr, c_int32_t, pdmd_check_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sys_id
+      type(c_ptr), value :: sv
+      type(pdmd_check_t), intent(out) :: out
     end function
     integer(c_int) function pdmd_stats(ctx, sys_id, out) bind(C, name='pdmd_stats')
       import :: c_int, c_ptr, c_int32_t, pdmd_stats_t

@@ -20,6 +20,9 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
 
+NO_HBS = 1  # pdmd_system_desc.flags: build flag -Dno_hbs
+
+
 class SystemDesc(ctypes.Structure):
     _fields_ = [
         ("nchains", ctypes.c_int32),
@@ -32,6 +35,8 @@ class SystemDesc(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("canon", ctypes.c_int32),
         ("list_capacity", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
         ("sv", _f64p),
         ("bptnr", _i32p),
     ]
@@ -104,6 +109,20 @@ class Obs(ctypes.Structure):
     ]
 
 
+class Check(ctypes.Structure):
+    _fields_ = [
+        ("overlaps", ctypes.c_int64),
+        ("bonds_long", ctypes.c_int64),
+        ("bonds_short", ctypes.c_int64),
+        ("first_i", ctypes.c_int32),
+        ("first_j", ctypes.c_int32),
+        ("boundbad", ctypes.c_int64),
+        ("unboundbad", ctypes.c_int64),
+        ("ss_counted", ctypes.c_int64),
+        ("ss_total", ctypes.c_int64),
+    ]
+
+
 EXPORTS = [
     "pdmd_version", "pdmd_last_error", "pdmd_ctx_create", "pdmd_ctx_destroy", "pdmd_system_add",
     "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
@@ -111,7 +130,7 @@ EXPORTS = [
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
     "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
-    "pdmd_set_waves", "pdmd_autotune_waves",
+    "pdmd_set_waves", "pdmd_autotune_waves", "pdmd_check",
 ]
 
 _lib = None
@@ -138,6 +157,7 @@ def lib():
         L.pdmd_wait.argtypes = [ctypes.c_void_p]
         L.pdmd_last_launch_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.pdmd_set_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.pdmd_check.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, ctypes.POINTER(Check)]
         L.pdmd_autotune_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, ctypes.c_int64, _i32p, _f64p]
         L.pdmd_download.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
         L.pdmd_stats.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Stats)]
@@ -202,9 +222,9 @@ def generate(chain_len, density=0.01, tstar=0.20, seed=1, param_dir=PARAM_DIR, s
             "density": density}
 
 
-def pair_codes(system, a, b, species=None):
+def pair_codes(system, a, b, species=None, no_hbs=False):
     """Static pair codes ev_code(a, b) (1-based beads) of a system; host only."""
-    d, keep = Engine.make_desc(system, 0, species=species)
+    d, keep = Engine.make_desc(system, 0, species=species, no_hbs=no_hbs)
     a = np.ascontiguousarray(a, np.int32)
     b = np.ascontiguousarray(b, np.int32)
     out = np.zeros(a.size, np.int32)
@@ -228,6 +248,14 @@ class Engine:
         """Wavefronts per system for later advances (pdmd_set_waves)."""
         _check(lib().pdmd_set_waves(self._ctx, int(waves)))
 
+    def check(self, sid, sv=None):
+        """checkover.f / check_nc_int.f on the device (pdmd_check): dict of
+        counts; sv replaces the resident positions/velocities when given."""
+        out = Check()
+        keep = None if sv is None else np.ascontiguousarray(sv, np.float64)
+        _check(lib().pdmd_check(self._ctx, sid, None if keep is None else _p(keep, ctypes.c_double), ctypes.byref(out)))
+        return {n: int(getattr(out, n)) for n, _ in Check._fields_}
+
     def autotune_waves(self, ids, probe_events=2000):
         """Pick the wavefronts per system by timing short advances of the
         listed systems (pdmd_autotune_waves); returns (waves, {waves: rate})."""
@@ -250,7 +278,7 @@ class Engine:
             pass
 
     @staticmethod
-    def make_desc(system, seed, canon=True, bptnr=None, species=None, list_capacity=0):
+    def make_desc(system, seed, canon=True, bptnr=None, species=None, list_capacity=0, no_hbs=False):
         keep = dict(cl=np.ascontiguousarray(system["chain_len"], np.int32),
                     rt=np.ascontiguousarray(system["restype"], np.int32),
                     hp=np.ascontiguousarray(system["hp"], np.int32),
@@ -260,11 +288,11 @@ class Engine:
         d = SystemDesc(len(keep["cl"]), _p(keep["cl"], ctypes.c_int32), _p(keep["sp"], ctypes.c_int32),
                        _p(keep["rt"], ctypes.c_int32), _p(keep["hp"], ctypes.c_int32), float(system["boxl"]),
                        float(system["tstar"]), int(seed), 1 if canon else 0, int(list_capacity),
-                       _p(keep["sv"], ctypes.c_double), _p(keep["bp"], ctypes.c_int32))
+                       NO_HBS if no_hbs else 0, 0, _p(keep["sv"], ctypes.c_double), _p(keep["bp"], ctypes.c_int32))
         return d, keep
 
-    def add_system(self, system, seed, canon=True, bptnr=None, species=None, list_capacity=0):
-        d, keep = self.make_desc(system, seed, canon, bptnr, species, list_capacity)
+    def add_system(self, system, seed, canon=True, bptnr=None, species=None, list_capacity=0, no_hbs=False):
+        d, keep = self.make_desc(system, seed, canon, bptnr, species, list_capacity, no_hbs)
         sid = ctypes.c_int32(0)
         _check(lib().pdmd_system_add(self._ctx, ctypes.byref(d), ctypes.byref(sid)))
         self._keep[sid.value] = keep

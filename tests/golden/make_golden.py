@@ -15,6 +15,7 @@ and stores inputs and outputs as .npz (data only).  Run in this container
 after `make -C oracle/ref`; the GPU box never runs it.
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -141,6 +142,37 @@ def read_outputs(work, N, L1, L2):
     return o
 
 
+def read_checks(stdout):
+    """Messages of checkover.f / check_nc_int.f between the harness's CHECK
+    and CHECKEND lines: per section the violations in message order as
+    (kind, i, j), kind 0 overlap, 1 bond too long, 2 bond too short; the
+    final section also gives over, boundbad, unboundbad and the count of
+    'no ss' messages."""
+    out, sec, rows, noss = {}, None, [], 0
+    pat = [(0, re.compile(r"particles\s+(\d+)\s+and\s+(\d+)\s+overlap")),
+           (1, re.compile(r"bond between particles\s+(\d+)\s+and\s+(\d+)\s+are too long")),
+           (2, re.compile(r"bond between particles\s+(\d+)\s+and\s+(\d+)\s+are too short"))]
+    for line in stdout.splitlines():
+        if line.startswith("CHECK "):
+            sec, rows, noss = line.split()[1], [], 0
+        elif line.startswith("CHECKEND") and sec:
+            f = line.split()[1:]
+            out["chk_" + sec] = np.array(rows, np.int32).reshape(-1, 3)
+            if sec == "final":
+                out["chk_final_flags"] = np.array([f[0] == "T", int(f[1]), int(f[2]), noss], np.int32)
+            else:
+                out["chk_" + sec + "_over"] = np.int32(f[0] == "T")
+            sec = None
+        elif sec:
+            if "no ss" in line:
+                noss += 1
+            for kind, rx in pat:
+                m = rx.search(line)
+                if m:
+                    rows.append((kind, int(m.group(1)), int(m.group(2))))
+    return out
+
+
 def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000):
     import pdmd
     rng = np.random.default_rng(seed)
@@ -159,6 +191,7 @@ def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000):
     print(name, r.stdout.strip().splitlines()[-1])
     N = 4 * (NCH1 * L1 + NCH2 * L2)
     o = read_outputs(work, N, L1, L2)
+    o.update(read_checks(r.stdout))
     o.update(chain_len=np.array([L1] * NCH1 + [L2] * NCH2, np.int32), restype=np.array(seq, np.int32),
              hp=sysd["hp"], boxl=sysd["boxl"], tstar=tstar, density=density, species=np.array([0] * NCH1 + [1] * NCH2,
                                                                                                np.int32))

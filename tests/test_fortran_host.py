@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "parallel-dmd-for-biomolecules_amd", "host")
 DRIVER = os.path.join(HOST, "pdmd_driver")
 NML = "&pdmd_run\n mode='{mode}', nchains1=3, chnln1=8, nchains2=2, chnln2=6, density=0.01, gen_seed=5,\n" \
-      " canon=1, seed=3, start='helix', waves={waves} /\n"
+      " canon=1, seed=3, start='helix', waves={waves}, debugging=1 /\n"
 
 
 @pytest.fixture(scope="module")

@@ -140,3 +140,35 @@ def test_serial_loop(param_dir):
         assert s.nupdates == fx["loop_updates"] and s.nforced == fx["loop_forced"], name
         assert s.tfalse == fx["tfalse1"], name
         _obs_close(orc.observables(), fx["obs_f"][1], fx["obs_i"][1])
+        _checks_match(orc.check, fx, name)
+
+
+def planted_states(fx):
+    """The two states the reference's checkover.f ran on (ref_harness.F90):
+    bead 5 put 1e-3 box lengths off bead N-3 on each axis, bead 10 moved by
+    0.05 box lengths along x."""
+    sv = np.array(fx["sv1"], np.float64).reshape(-1, 6)
+    N = sv.shape[0]
+    p1 = sv.copy()
+    p1[4, :3] = sv[N - 4, :3] + 1.0e-3
+    p2 = sv.copy()
+    p2[9, 0] = sv[9, 0] + 5.0e-2
+    return {"planted1": p1.ravel(), "planted2": p2.ravel()}
+
+
+def _checks_match(check, fx, name):
+    """Counts by kind and first pair of a check function against the
+    reference's checkover.f / check_nc_int.f messages."""
+    got = check()
+    over, boundbad, unboundbad, noss = (int(x) for x in fx["chk_final_flags"])
+    assert (got["overlaps"] + got["bonds_long"] + got["bonds_short"] > 0) == bool(over), name
+    assert fx["chk_final"].shape[0] == got["overlaps"] + got["bonds_long"] + got["bonds_short"], name
+    assert got["boundbad"] == boundbad and got["unboundbad"] == unboundbad, name
+    assert got["ss_total"] - got["ss_counted"] == 0 and noss == 0, name
+    for sec, sv in planted_states(fx).items():
+        got = check(sv)
+        rows = fx["chk_" + sec]
+        want = [int((rows[:, 0] == k).sum()) for k in range(3)]
+        assert [got["overlaps"], got["bonds_long"], got["bonds_short"]] == want, (name, sec, got, want)
+        assert (got["first_i"], got["first_j"]) == (int(rows[0, 1]), int(rows[0, 2])), (name, sec)
+        assert bool(fx["chk_" + sec + "_over"]), name
