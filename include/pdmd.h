@@ -166,7 +166,10 @@ int pdmd_predict_pairs(pdmd_ctx* ctx, int32_t sys_id, int32_t n, const int32_t* 
  * per-bead topology words instead of the N x N int8 matrix. */
 int pdmd_pair_codes(const pdmd_system_desc* desc, int32_t n, const int32_t* a, const int32_t* b, int32_t* code);
 
-/* Observables of the output pseudo-event (energy.f, radgyr.f, end2end.f). */
+/* Observables of the output pseudo-event (energy.f:25-101, radgyr.f:18-135,
+ * end2end.f:14-44; main.F90:1191-1246), reduced on the device over the
+ * resident state at the current tfalse: side-chain well energies over the
+ * neighbour lists instead of energy.f's all-pairs loop. */
 int pdmd_observables(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
 
 typedef struct {

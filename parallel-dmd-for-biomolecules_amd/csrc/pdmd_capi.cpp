@@ -830,121 +830,36 @@ int pdmd_check(pdmd_ctx* ctx, int32_t id, const double* sv, pdmd_check_t* out) {
 
 int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
     HostSys* H = get_sys(ctx, id);
-    if (!H || !o) return -1;
-    const int N = H->N;
-    std::vector<double> sv(6 * (size_t)N);
-    std::vector<int> ident(N), bp(N);
-    if (int e = pdmd_download(ctx, id, sv.data(), ident.data(), bp.data(), nullptr, nullptr, nullptr, nullptr)) return e;
-    SysState st;
-    HIPCHK(hipMemcpy(&st, H->st.p, sizeof st, hipMemcpyDeviceToHost));
-    const double tf = st.tfalse;
+    if (!H || !o) return fail("pdmd_observables: bad arguments");
     const Topology& t = H->topo;
-    const Params& P = H->p;
-    std::memset(o, 0, sizeof *o);
-    // energy.f:25-101
-    int hb_ii = 0, hb_ij = 0, hb_alpha = 0;
-    for (int i = 0; i < N; ++i) {
-        const int j = bp[i] - 1;
-        if (j > i) (t.chain[i] == t.chain[j] ? hb_ii : hb_ij)++;
-    }
-    double ehh_ii = 0.0, ehh_ij = 0.0;
-    auto pos = [&](int k, int c) { return sv[6 * k + c] + sv[6 * k + 3 + c] * tf; };
-    std::vector<int> side;
-    for (int k = 0; k < N; ++k)
-        if ((t.loc[k] - 1) / t.chain_len[t.chain[k]] == 3) side.push_back(k);
-    for (size_t a = 0; a < side.size(); ++a)
-        for (size_t b = a + 1; b < side.size(); ++b) {
-            const int i = side[a], j = side[b];
-            if (bp[i] - 1 == j) continue;
-            if (static_code(t, j, i) != 16) continue;
-            double r[3];
-            for (int c = 0; c < 3; ++c) {
-                const double v = sv[6 * i + 3 + c] - sv[6 * j + 3 + c];
-                double x = sv[6 * i + c] - sv[6 * j + c] + v * tf;
-                r[c] = x - std::round(x);
-            }
-            const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            if (r2 <= P.welldia_sq[ident[i]][ident[j]]) {
-                const double dep = P.ep_sqrt[ident[i]][ident[j]];
-                (t.chain[i] == t.chain[j] ? ehh_ii : ehh_ij) += dep;
-            }
-        }
-    for (size_t c = 0; c < t.chain_len.size(); ++c) {
-        const int s = t.chain_start[c], L = t.chain_len[c];
-        for (int i = s + L + 4; i < s + 2 * L; ++i)
-            if (bp[i] - 1 == i + L - 4) ++hb_alpha;
-    }
-    const double eps_hb = P.ep_sqrt[5][8];
-    const int sum_hb = hb_ii + hb_ij;
-    const double sumeps = -(sum_hb * eps_hb + (ehh_ii + ehh_ij));
-    double sumvel = 0.0;
-    for (int i = 0; i < N; ++i) {
-        const double m = ctx->raw.bmass[t.ident0[i]];
-        sumvel = sumvel + m * (sv[6 * i + 3] * sv[6 * i + 3] + sv[6 * i + 4] * sv[6 * i + 4] + sv[6 * i + 5] * sv[6 * i + 5]);
-    }
-    o->ered = 0.5 * sumvel + sumeps;
-    o->tred = sumvel / 3.0 / (double)N;
-    o->sumvel = sumvel;
-    o->hb_alpha = hb_alpha;
-    o->hb_ii = hb_ii;
-    o->hb_ij = hb_ij;
-    o->ehh_ii = ehh_ii;
-    o->ehh_ij = ehh_ij;
-    // radgyr.f:18-135 (backbone beads, chains unwrapped along the backbone)
-    double rg = 0.0, e2e = 0.0;
-    int nbb = 0;
     const int nch = (int)t.chain_len.size();
     int nfirst = 0;  // chains of the leading species block: the reference's species 1
     while (nfirst < nch && t.species[nfirst] == t.species[0]) nfirst++;
-    std::vector<double> q(3 * (size_t)t.N);
-    for (int c = 0; c < nch; ++c) {
-        const int s = t.chain_start[c], L = t.chain_len[c];
-        for (int k = s; k < s + 4 * L; ++k)
-            for (int d3 = 0; d3 < 3; ++d3) q[3 * k + d3] = pos(k, d3);
-        auto unwrap = [&](int a, int b) {
-            for (int d3 = 0; d3 < 3; ++d3) {
-                const double dd = q[3 * a + d3] - q[3 * b + d3];
-                if (dd > 0.5) q[3 * a + d3] = q[3 * a + d3] - 1.0;
-                else if (dd < -1.0 * 0.5) q[3 * a + d3] = q[3 * a + d3] + 1.0;
-            }
-        };
-        for (int k = s + 1; k < s + L; ++k) unwrap(k, k - 1);
-        for (int k = s; k < s + L; ++k) unwrap(L + k, k);
-        for (int k = s; k < s + L; ++k) unwrap(2 * L + k, k);
-        // radgyr.f:117-125 sums a species-2 chain i over beads
-        // (i-1)*numbeads2+1 .. (i-1)*numbeads2+3*chnln2, without the nop1
-        // offset its unwrapping uses (:78); reproduced while those beads are
-        // already filled, the chain's own beads otherwise
-        int rb = s;
-        if (c >= nfirst && c * 4 * L + 3 * L <= s + 4 * L) rb = c * 4 * L;
-        double cx = 0, cy = 0, cz = 0, rgs = 0;
-        for (int k = rb; k < rb + 3 * L; ++k) {
-            cx = cx + q[3 * k];
-            cy = cy + q[3 * k + 1];
-            cz = cz + q[3 * k + 2];
-        }
-        cx = cx / (double)(3 * L);
-        cy = cy / (double)(3 * L);
-        cz = cz / (double)(3 * L);
-        for (int k = rb; k < rb + 3 * L; ++k) {
-            const double rx = (q[3 * k] - cx) * (q[3 * k] - cx), ry = (q[3 * k + 1] - cy) * (q[3 * k + 1] - cy),
-                         rz = (q[3 * k + 2] - cz) * (q[3 * k + 2] - cz);
-            rgs = rgs + rx + ry + rz;
-        }
-        rg = rg + rgs;
-        nbb += 3 * L;
-        // end2end.f: N of residue 1 to C of residue L
-        double e[3];
-        const int a = s + L, b = s + 3 * L - 1;
-        for (int d3 = 0; d3 < 3; ++d3) {
-            const double v = sv[6 * a + 3 + d3] - sv[6 * b + 3 + d3];
-            const double x = sv[6 * a + d3] - sv[6 * b + d3] + v * tf;
-            e[d3] = x - std::round(x);
-        }
-        e2e = e2e + std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-    }
-    o->rg_avg = std::sqrt(rg / (double)nbb) * P.boxl_orig;
-    o->e2e_avg = e2e / (double)nch * P.boxl_orig;
+    DevBuf<int> cs, cl, oi;
+    DevBuf<double> od;
+    HIPCHK(cs.alloc(nch));
+    HIPCHK(cl.alloc(nch));
+    HIPCHK(od.alloc(7));
+    HIPCHK(oi.alloc(3));
+    HIPCHK(hipMemcpyAsync(cs.p, t.chain_start.data(), sizeof(int) * nch, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(cl.p, t.chain_len.data(), sizeof(int) * nch, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(engine_observables(ctx->d_sys.p, id - 1, cs.p, cl.p, nch, nfirst, H->p.boxl_orig, od.p, oi.p, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double d[7];
+    int n[3];
+    HIPCHK(hipMemcpy(d, od.p, sizeof d, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(n, oi.p, sizeof n, hipMemcpyDeviceToHost));
+    std::memset(o, 0, sizeof *o);
+    o->ered = d[0];
+    o->tred = d[1];
+    o->sumvel = d[2];
+    o->ehh_ii = d[3];
+    o->ehh_ij = d[4];
+    o->rg_avg = d[5];
+    o->e2e_avg = d[6];
+    o->hb_alpha = n[0];
+    o->hb_ii = n[1];
+    o->hb_ij = n[2];
     return 0;
 }
 

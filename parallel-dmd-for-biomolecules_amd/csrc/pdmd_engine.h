@@ -29,6 +29,12 @@ hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_
 // counters (zeroed by the caller, out[3] preset to all ones); alt replaces
 // the resident state records when not null.
 hipError_t engine_check(const SysDev* d_sys, int id, int n, const Rec* alt, unsigned long long* out, hipStream_t s);
+// Observables of system id (energy.f, radgyr.f, end2end.f) on one
+// workgroup: outd[7] ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg;
+// outi[3] hb_alpha, hb_ii, hb_ij.  cstart/clen: nch chains (device arrays);
+// nfirst: chains of the leading species block; boxl in Angstrom.
+hipError_t engine_observables(const SysDev* d_sys, int id, const int* cstart, const int* clen, int nch, int nfirst,
+                              double boxl, double* outd, int* outi, hipStream_t s);
 // Test hook: collision times of (i, j) pairs (1-based) of system id.
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s);

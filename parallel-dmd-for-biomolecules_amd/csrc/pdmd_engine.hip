@@ -52,12 +52,14 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(th
 // copies, 20 neighbour search; multi-wavefront rounds: 21 rounds of two or
 // more pair events, 22 entries popped in them, 23 entries committed, 24
 // footprint marks up to the first round barrier, 25 footprint checks up to
-// the second, 26 new keys up to the third, 27 the round's rescans and merge.
+// the second, 26 new keys up to the third, 27 the round's rescans and merge,
+// 28 rounds cut by a footprint conflict or an event that cannot speculate,
+// 29 rounds cut by the key order.
 enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
        kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
        kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20,
        kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23, kPhBMark = 24, kPhBCheck = 25, kPhBKeys = 26,
-       kPhBTail = 27 };
+       kPhBTail = 27, kPhNCutMark = 28, kPhNCutKey = 29 };
 #ifdef PDMD_PHASE_TIMERS
 #define PH_BEGIN(v) long long v = clock64()
 #define PH_END(S, k, v)                                                                                  \
@@ -3085,6 +3087,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             PH_COUNT(S, kPhNRound);
             PH_ADDN(S, kPhNPopped, K);
             PH_ADDN(S, kPhNCommit, A);
+            PH_ADDN(S, kPhNCutMark, min(K, B.cut) < K ? 1 : 0);
+            PH_ADDN(S, kPhNCutKey, A < min(K, B.cut) ? 1 : 0);
         }
         __syncthreads();
         // block minima of every committed event's blocks and of the popped
@@ -3232,6 +3236,152 @@ __global__ void k_check(const SysDev* sys, int id, const Rec* alt, unsigned long
     if (mss) atomicAdd(out + 7, mss);
 }
 
+// ------------------------------------------------------------------------
+// observables of one system: energy.f:25-101, radgyr.f:18-135,
+// end2end.f:14-44, on one workgroup of kObsT lanes
+// ------------------------------------------------------------------------
+constexpr int kObsT = 256;
+// sum over the workgroup in a fixed tree (the same result on every call)
+__device__ inline double obs_sum(double v, double* red) {
+    const int t = (int)threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int w = kObsT / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] = red[t] + red[t + w];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+// Bead sums in a fixed order per lane (kinetic energy, H-bond counts,
+// side-chain well energies over the up lists: every side-chain pair inside
+// a well is listed, wells lie inside the list cutoff); chain walks one lane
+// per chain (backbone unwrapping into q, then radius of gyration and
+// end-to-end distance).  nfirst: chains of the leading species block.
+// outd: ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg; outi: hb_alpha,
+// hb_ii, hb_ij.
+__global__ __launch_bounds__(kObsT) void k_obs(const SysDev* sys, int id, const int* cstart, const int* clen, int nch,
+                                              int nfirst, double boxl, double* outd, int* outi) {
+    __shared__ double red[kObsT];
+    const SysDev& S = sys[id];
+    const DevTables& T = *S.tab;
+    const int N = S.N, cap = S.cap, t = (int)threadIdx.x;
+    const double tf = S.st->tfalse;
+    double sumvel = 0.0, eii = 0.0, eij = 0.0;
+    double hii = 0.0, hij = 0.0, halpha = 0.0;
+    for (int i = t; i < N; i += kObsT) {
+        const Rec r = S.rec[i];
+        sumvel = sumvel + r.bm * (r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
+        const int j = dyn_bptnr(r.dyn);
+        if (j > i) {
+            if (topo_chain(r.topo) == topo_chain(S.rec[j].topo))
+                hii = hii + 1.0;
+            else
+                hij = hij + 1.0;
+        }
+        const int kind = topo_kind(r.topo), res = topo_res(r.topo), L = topo_len(r.topo);
+        if (kind == 1 && res >= 5 && j == i + L - 4) halpha = halpha + 1.0;
+        if (kind == 3) {
+            const int nu = min(S.nup[i], cap), idi = dyn_ident(r.dyn);
+            for (int k = 0; k < nu; ++k) {
+                const int ent = S.nb_up[(size_t)i * cap + k];
+                const int m = ent & 0xFFFFFF;
+                if ((ent >> 24) != 16 || m == j) continue;
+                const Rec rm = S.rec[m];
+                Geo g;
+                geo(r, rm, tf, g);
+                const int idm = dyn_ident(rm.dyn);
+                if (g.r2 <= T.welldia_sq[idi][idm]) {
+                    if (topo_chain(r.topo) == topo_chain(rm.topo))
+                        eii = eii + T.ep_sqrt[idi][idm];
+                    else
+                        eij = eij + T.ep_sqrt[idi][idm];
+                }
+            }
+        }
+    }
+    // backbone unwrapping along each chain (radgyr.f:48-104) into q
+    double4* q = S.spos;
+    for (int c = t; c < nch; c += kObsT) {
+        const int s = cstart[c], L = clen[c];
+        for (int k = s; k < s + 3 * L; ++k) {
+            const Rec b = S.rec[k];
+            q[k] = make_double4(b.x + b.vx * tf, b.y + b.vy * tf, b.z + b.vz * tf, 0.0);
+        }
+        auto unwrap = [&](int a, int b) {
+            double4 qa = q[a];
+            const double4 qb = q[b];
+            double dd = qa.x - qb.x;
+            if (dd > 0.5) qa.x = qa.x - 1.0;
+            else if (dd < -1.0 * 0.5) qa.x = qa.x + 1.0;
+            dd = qa.y - qb.y;
+            if (dd > 0.5) qa.y = qa.y - 1.0;
+            else if (dd < -1.0 * 0.5) qa.y = qa.y + 1.0;
+            dd = qa.z - qb.z;
+            if (dd > 0.5) qa.z = qa.z - 1.0;
+            else if (dd < -1.0 * 0.5) qa.z = qa.z + 1.0;
+            q[a] = qa;
+        };
+        for (int k = s + 1; k < s + L; ++k) unwrap(k, k - 1);
+        for (int k = s; k < s + L; ++k) unwrap(L + k, k);
+        for (int k = s; k < s + L; ++k) unwrap(2 * L + k, k);
+    }
+    __threadfence_block();
+    __syncthreads();
+    double rg = 0.0, e2e = 0.0, nbb = 0.0;
+    for (int c = t; c < nch; c += kObsT) {
+        const int s = cstart[c], L = clen[c];
+        // radgyr.f:117-125 sums a chain of the second species block over
+        // beads (c-1)*numbeads2+1 .. +3*chnln2 without the first block's
+        // offset its unwrapping uses (:78), while those beads exist
+        int rb = s;
+        if (c >= nfirst && c * 4 * L + 3 * L <= s + 4 * L) rb = c * 4 * L;
+        double cx = 0, cy = 0, cz = 0, rgs = 0;
+        for (int k = rb; k < rb + 3 * L; ++k) {
+            const double4 v = q[k];
+            cx = cx + v.x;
+            cy = cy + v.y;
+            cz = cz + v.z;
+        }
+        cx = cx / (double)(3 * L);
+        cy = cy / (double)(3 * L);
+        cz = cz / (double)(3 * L);
+        for (int k = rb; k < rb + 3 * L; ++k) {
+            const double4 v = q[k];
+            const double rx = (v.x - cx) * (v.x - cx), ry = (v.y - cy) * (v.y - cy), rz = (v.z - cz) * (v.z - cz);
+            rgs = rgs + rx + ry + rz;
+        }
+        rg = rg + rgs;
+        nbb = nbb + (double)(3 * L);
+        e2e = e2e + dist_rec(S.rec[s + L], S.rec[s + 3 * L - 1], tf);  // end2end.f: N of residue 1 to C of residue L
+    }
+    sumvel = obs_sum(sumvel, red);
+    eii = obs_sum(eii, red);
+    eij = obs_sum(eij, red);
+    hii = obs_sum(hii, red);
+    hij = obs_sum(hij, red);
+    halpha = obs_sum(halpha, red);
+    rg = obs_sum(rg, red);
+    e2e = obs_sum(e2e, red);
+    nbb = obs_sum(nbb, red);
+    if (t == 0) {
+        const double eps_hb = T.ep_sqrt[5][8];
+        const int sum_hb = (int)hii + (int)hij;
+        const double sumeps = -(sum_hb * eps_hb + (eii + eij));
+        outd[0] = 0.5 * sumvel + sumeps;
+        outd[1] = sumvel / 3.0 / (double)N;
+        outd[2] = sumvel;
+        outd[3] = eii;
+        outd[4] = eij;
+        outd[5] = sqrt(rg / nbb) * boxl;
+        outd[6] = e2e / (double)nch * boxl;
+        outi[0] = (int)halpha;
+        outi[1] = (int)hii;
+        outi[2] = (int)hij;
+    }
+}
+
 // test hook: predicted collision time of listed pairs (1-based in, code
 // from ev_code(j,i) as events.f/eventredo_up.f read it)
 __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const int* pj, double* tij, int* type) {
@@ -3304,6 +3454,11 @@ hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, con
 }
 hipError_t engine_check(const SysDev* d_sys, int id, int n, const Rec* alt, unsigned long long* out, hipStream_t s) {
     hipLaunchKernelGGL(k_check, dim3((n + 255) / 256), dim3(256), 0, s, d_sys, id, alt, out);
+    return hipGetLastError();
+}
+hipError_t engine_observables(const SysDev* d_sys, int id, const int* cstart, const int* clen, int nch, int nfirst,
+                              double boxl, double* outd, int* outi, hipStream_t s) {
+    hipLaunchKernelGGL(k_obs, dim3(1), dim3(kObsT), 0, s, d_sys, id, cstart, clen, nch, nfirst, boxl, outd, outi);
     return hipGetLastError();
 }
 hipError_t engine_prepare(size_t lds_limit) {

@@ -50,3 +50,23 @@ def test_device_matches_reference_serial_loop(name, no_hbs, waves):
     assert np.array_equal(gs["sv"].view(np.int64), np.ascontiguousarray(fx["sv1"]).view(np.int64))
     assert np.array_equal(gs["nptnr"], fx["nptnr1"]) and np.array_equal(gs["coltype"], fx["coltype1"])
     assert np.array_equal(gs["tim"].view(np.int64), np.ascontiguousarray(fx["tim1"]).view(np.int64))
+
+
+@pytest.mark.parametrize("name,no_hbs", [("ref_2species", False), ("ref_2species_nohbs", True)])
+def test_device_observables_match_reference(name, no_hbs):
+    """energy.f / radgyr.f / end2end.f on the device against the reference's
+    own values before and after the serial loop (1e-12 relative: sums run
+    in another order)."""
+    from test_oracle_reference import _obs_close
+
+    fx = dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
+    sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"],
+                boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
+    eng = pdmd.Engine()
+    try:
+        sid = eng.add_system(sysd, seed=1, canon=False, species=fx["species"], no_hbs=no_hbs)
+        _obs_close(eng.observables(sid), fx["obs_f"][0], fx["obs_i"][0])
+        eng.advance([sid], [int(fx["loop_n"])])
+        _obs_close(eng.observables(sid), fx["obs_f"][1], fx["obs_i"][1])
+    finally:
+        eng.close()
