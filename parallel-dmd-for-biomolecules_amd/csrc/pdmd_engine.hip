@@ -3305,7 +3305,9 @@ __global__ __launch_bounds__(kObsT) void k_obs(const SysDev* sys, int id, const 
     double4* q = S.spos;
     for (int c = t; c < nch; c += kObsT) {
         const int s = cstart[c], L = clen[c];
-        for (int k = s; k < s + 3 * L; ++k) {
+        // every bead of the chain: the second species' sums below can reach
+        // into an earlier chain's side-chain beads
+        for (int k = s; k < s + 4 * L; ++k) {
             const Rec b = S.rec[k];
             q[k] = make_double4(b.x + b.vx * tf, b.y + b.vy * tf, b.z + b.vz * tf, 0.0);
         }
