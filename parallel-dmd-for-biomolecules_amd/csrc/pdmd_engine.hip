@@ -2911,9 +2911,18 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
 
     Batch& B = *L.batch();
     const int lane = lane_id();
+    // the views this loop uses, read once (wave fences below would make the
+    // compiler read them from LDS again)
+    Key* const blk = L.blk();
+    Key* const sup = L.sup();
+    const int nsup = L.nsup, nblk = L.nblk;
+    const bool leaf_lds = L.leaf_lds;
+    const Leaf* const lleaf = L.lleaf();
+    const Leaf* const gleaf = L.gleaf;
     int K = 0;
+    unsigned pip = 0xFFFFFFFFu;  // lane m: index << 16 | partner of popped entry m
     while (true) {
-        const Key k = key_min_wave(lane < L.nsup ? L.sup()[lane] : empty_key());
+        const Key k = key_min_wave(lane < nsup ? sup[lane] : empty_key());
         if (k.bc == kEmptyBc) {
             if (K == 0) c.status = kErrBin;
             break;
@@ -2921,22 +2930,24 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
         const int e = key_idx(k);
         if (K > 0 && e >= N) break;
         if (lane == 0) B.key[K] = k;
+        if (lane == K) pip = k.ip;
         ++K;
         if (e >= N || K >= kmax) break;
-        wave_sync();
+        // take e out of the tree: its block without the popped entries, then
+        // its superblock
         const int b = e / kBlk, sp = b / kSup;
         Key x = empty_key();
         const int idx = b * kBlk + (lane & 15);
         if (lane < kBlk && idx < N3) {
             bool popped = false;
-            for (int m = 0; m < K; ++m) popped |= key_idx(B.key[m]) == idx;
-            if (!popped) x = leaf_key(leaf_get(L, idx), idx);
+            for (int m = 0; m < K; ++m) popped |= (int)(__builtin_amdgcn_readlane((int)pip, m) >> 16) == idx;
+            if (!popped) x = leaf_key(leaf_lds ? lleaf[idx] : gld(gleaf + idx), idx);
         }
-        if (key_min_rows(x) && lane < kBlk) L.blk()[b] = x;
+        if (key_min_rows(x) && lane < kBlk) blk[b] = x;
         wave_sync();
         const int bb = sp * kSup + lane;
-        const Key y = key_min_wave(bb < L.nblk ? L.blk()[bb] : empty_key());
-        if (lane == 0) L.sup()[sp] = y;
+        const Key y = key_min_wave(bb < nblk ? blk[bb] : empty_key());
+        if (lane == 0) sup[sp] = y;
         wave_sync();
     }
     wave_sync();
