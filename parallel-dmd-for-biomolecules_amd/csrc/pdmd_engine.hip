@@ -1261,12 +1261,23 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const DevTables& T = tabs();
     Sc& c = *L.sc();
     const Pair& P = *L.pair();
+    // the wavefront's work-area views, held in scalar registers: the wave
+    // fences below would make the compiler read them from LDS again
+    int* const v_wm = reinterpret_cast<int*>(smem + __builtin_amdgcn_readfirstlane(L.o_wm));
+    int* const v_wx = reinterpret_cast<int*>(smem + __builtin_amdgcn_readfirstlane(L.o_wx));
+    int* const v_wd = reinterpret_cast<int*>(smem + __builtin_amdgcn_readfirstlane(L.o_wd));
+    int* const v_wy = reinterpret_cast<int*>(smem + __builtin_amdgcn_readfirstlane(L.o_wy));
+    double* const v_wt = reinterpret_cast<double*>(smem + __builtin_amdgcn_readfirstlane(L.o_wt));
+    MinSlot* const v_slot = reinterpret_cast<MinSlot*>(smem + __builtin_amdgcn_readfirstlane(L.o_slot));
+    SpecInfo* const v_spec = reinterpret_cast<SpecInfo*>(smem + __builtin_amdgcn_readfirstlane(L.o_spec));
+    DownInfo* const v_down = reinterpret_cast<DownInfo*>(smem + __builtin_amdgcn_readfirstlane(L.o_down));
+    const int maxw = __builtin_amdgcn_readfirstlane(L.maxw);
 
     const int lane = lane_id(), cap = S.cap, N3 = S.N + 3;
     const int i = P.i, j = P.j;
     const double tf = c.tfalse;
     const double T0 = c.interval_max + kLt - tf;
-    const int dbase = L.maxw - (2 * cap + 8);  // down-set region of the work list
+    const int dbase = maxw - (2 * cap + 8);  // down-set region of the work list
     const bool batch = bk >= 0;
     int status = 0, nb = 0;
     PH_BEGIN(tph);
@@ -1284,48 +1295,48 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     // ---- work list: candidates of i (owner 0) and j (owner 1) at 0..
     int w = 0;
     if (lane < nui) {
-        L.wm()[lane] = eui;
-        L.wx()[lane] = 0;
+        v_wm[lane] = eui;
+        v_wx[lane] = 0;
     }
     w = nui;
-    if (xi0 > i) { if (lane == 0) { L.wm()[w] = xi0 | kNoStat; L.wx()[w] = 0; } ++w; }
-    if (xi1 > i) { if (lane == 0) { L.wm()[w] = xi1 | kNoStat; L.wx()[w] = 0; } ++w; }
-    if (xi2 > i) { if (lane == 0) { L.wm()[w] = xi2 | kNoStat; L.wx()[w] = 0; } ++w; }
+    if (xi0 > i) { if (lane == 0) { v_wm[w] = xi0 | kNoStat; v_wx[w] = 0; } ++w; }
+    if (xi1 > i) { if (lane == 0) { v_wm[w] = xi1 | kNoStat; v_wx[w] = 0; } ++w; }
+    if (xi2 > i) { if (lane == 0) { v_wm[w] = xi2 | kNoStat; v_wx[w] = 0; } ++w; }
     if (j >= 0) {
         if (lane < nuj) {
-            L.wm()[w + lane] = euj;
-            L.wx()[w + lane] = 1;
+            v_wm[w + lane] = euj;
+            v_wx[w + lane] = 1;
         }
         w += nuj;
-        if (xj0 > j) { if (lane == 0) { L.wm()[w] = xj0 | kNoStat; L.wx()[w] = 1; } ++w; }
-        if (xj1 > j) { if (lane == 0) { L.wm()[w] = xj1 | kNoStat; L.wx()[w] = 1; } ++w; }
-        if (xj2 > j) { if (lane == 0) { L.wm()[w] = xj2 | kNoStat; L.wx()[w] = 1; } ++w; }
+        if (xj0 > j) { if (lane == 0) { v_wm[w] = xj0 | kNoStat; v_wx[w] = 1; } ++w; }
+        if (xj1 > j) { if (lane == 0) { v_wm[w] = xj1 | kNoStat; v_wx[w] = 1; } ++w; }
+        if (xj2 > j) { if (lane == 0) { v_wm[w] = xj2 | kNoStat; v_wx[w] = 1; } ++w; }
     }
     const int ncand = w;
     // ---- the down set at dbase.., tag 2 (op with i) / 3 (op with j), in
     //      reference order: i's down list and lower shoulder partners, then j's
     int d = 0;
     if (lane < ndi) {
-        L.wm()[dbase + lane] = edi & 0xFFFFFF;
-        L.wx()[dbase + lane] = 2;
+        v_wm[dbase + lane] = edi & 0xFFFFFF;
+        v_wx[dbase + lane] = 2;
     }
     d = ndi;
-    if (xi0 >= 0 && xi0 < i) { if (lane == 0) { L.wm()[dbase + d] = xi0; L.wx()[dbase + d] = 2; } ++d; }
-    if (xi1 >= 0 && xi1 < i) { if (lane == 0) { L.wm()[dbase + d] = xi1; L.wx()[dbase + d] = 2; } ++d; }
-    if (xi2 >= 0 && xi2 < i) { if (lane == 0) { L.wm()[dbase + d] = xi2; L.wx()[dbase + d] = 2; } ++d; }
+    if (xi0 >= 0 && xi0 < i) { if (lane == 0) { v_wm[dbase + d] = xi0; v_wx[dbase + d] = 2; } ++d; }
+    if (xi1 >= 0 && xi1 < i) { if (lane == 0) { v_wm[dbase + d] = xi1; v_wx[dbase + d] = 2; } ++d; }
+    if (xi2 >= 0 && xi2 < i) { if (lane == 0) { v_wm[dbase + d] = xi2; v_wx[dbase + d] = 2; } ++d; }
     if (j >= 0) {
         const int l = edj & 0xFFFFFF;
         const bool keep = lane < ndj && l != i;
         const unsigned long long bal = __ballot(keep);
         const int pre = __popcll(bal & ((1ull << lane) - 1ull));
         if (keep) {
-            L.wm()[dbase + d + pre] = l;
-            L.wx()[dbase + d + pre] = 3;
+            v_wm[dbase + d + pre] = l;
+            v_wx[dbase + d + pre] = 3;
         }
         d += __popcll(bal);
-        if (xj0 >= 0 && xj0 < j) { if (lane == 0) { L.wm()[dbase + d] = xj0; L.wx()[dbase + d] = 3; } ++d; }
-        if (xj1 >= 0 && xj1 < j) { if (lane == 0) { L.wm()[dbase + d] = xj1; L.wx()[dbase + d] = 3; } ++d; }
-        if (xj2 >= 0 && xj2 < j) { if (lane == 0) { L.wm()[dbase + d] = xj2; L.wx()[dbase + d] = 3; } ++d; }
+        if (xj0 >= 0 && xj0 < j) { if (lane == 0) { v_wm[dbase + d] = xj0; v_wx[dbase + d] = 3; } ++d; }
+        if (xj1 >= 0 && xj1 < j) { if (lane == 0) { v_wm[dbase + d] = xj1; v_wx[dbase + d] = 3; } ++d; }
+        if (xj2 >= 0 && xj2 < j) { if (lane == 0) { v_wm[dbase + d] = xj2; v_wx[dbase + d] = 3; } ++d; }
     }
     const int nD = d;
     if (nD > kW) {
@@ -1345,8 +1356,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         return;
     }
     if (lane < 2 + kMaxSpec) {
-        L.slot()[lane].tkey = ~0ull;
-        L.slot()[lane].packed = ~0u;
+        v_slot[lane].tkey = ~0ull;
+        v_slot[lane].packed = ~0u;
     }
     wave_sync();
     PH_END(S, kPhRows, tph);
@@ -1354,11 +1365,11 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     //      bead's op counts, and whether its full recompute is batched
     int l = -1, cI = 0, cJ = 0;
     bool first = false;
-    const int dent = lane < nD ? L.wm()[dbase + lane] : -1;
+    const int dent = lane < nD ? v_wm[dbase + lane] : -1;
     {
         // every lane compares its bead with each entry, read from lanes
         const int lv = dent;
-        const int tv = lane < nD ? L.wx()[dbase + lane] : 0;
+        const int tv = lane < nD ? v_wx[dbase + lane] : 0;
         l = lv;
         first = lane < nD;
         for (int r = 0; r < nD; ++r) {
@@ -1386,7 +1397,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const int myslot = spec ? __popcll(smask & ((1ull << lane) - 1ull)) : -1;
     if (myslot >= kMaxSpec) spec = false;
     const int nspec = min(__popcll(smask), kMaxSpec);
-    if (spec) L.spec()[myslot].l = l;
+    if (spec) v_spec[myslot].l = l;
     wave_sync();
     PH_END(S, kPhScan, tph);
     // ---- round 2: the down beads' own data, the rows of the batched beads
@@ -1406,9 +1417,9 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
 #pragma unroll
     for (int s2 = 0; s2 < kMaxSpec; ++s2) {
         sent[s2] = 0;
-        if (s2 < nspec && lane < cap) sent[s2] = gl1(S.nb_up + ((size_t)L.spec()[s2].l * cap + lane));
+        if (s2 < nspec && lane < cap) sent[s2] = gl1(S.nb_up + ((size_t)v_spec[s2].l * cap + lane));
     }
-    const int cent = lane < ncand ? L.wm()[lane] : 0;
+    const int cent = lane < ncand ? v_wm[lane] : 0;
     const int cm = cent & 0xFFFFFF;
     Rec crm = P.ri;
     int4 com = make_int4(-1, -1, -1, -1);
@@ -1418,14 +1429,14 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     }
     // the down beads' own data for their single-pair predictions
     if (first && cI + cJ > 0) {
-        DownInfo& di = L.down()[lane];
+        DownInfo& di = v_down[lane];
         di.r = rl;
         di.o = ol;
         di.bw = bwl;
     }
     // the batched beads' pair entries, after the candidates
     if (spec) {
-        SpecInfo& si = L.spec()[myslot];
+        SpecInfo& si = v_spec[myslot];
         si.r = rl;
         si.bw = bwl;
         si.x = xl;
@@ -1436,7 +1447,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     if (lane == 0) {
         int off = ncand;
         for (int s2 = 0; s2 < nspec; ++s2) {
-            SpecInfo& si = L.spec()[s2];
+            SpecInfo& si = v_spec[s2];
             const int n = si.nu + si.nx;
             if (off + n <= dbase) {
                 si.off = off;
@@ -1444,26 +1455,26 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             } else
                 si.off = -1;  // no room: this bead's recompute runs on its own
         }
-        L.wd()[0] = off;
+        v_wd[0] = off;
     }
     wave_sync();
-    const int npairs = L.wd()[0];
+    const int npairs = v_wd[0];
 #pragma unroll
     for (int s2 = 0; s2 < kMaxSpec; ++s2) {
         if (s2 < nspec) {
-            const int off = L.spec()[s2].off, nu = L.spec()[s2].nu;
+            const int off = v_spec[s2].off, nu = v_spec[s2].nu;
             if (off >= 0) {
                 if (lane < nu) {
-                    L.wm()[off + lane] = sent[s2];
-                    L.wx()[off + lane] = 2 + s2;
+                    v_wm[off + lane] = sent[s2];
+                    v_wx[off + lane] = 2 + s2;
                 }
                 if (lane == 0) {
-                    const int4 x = L.spec()[s2].x;
-                    const int ls = L.spec()[s2].l;
+                    const int4 x = v_spec[s2].x;
+                    const int ls = v_spec[s2].l;
                     int k = off + nu;
-                    if (x.x > ls) { L.wm()[k] = x.x | kNoStat; L.wx()[k] = 2 + s2; ++k; }
-                    if (x.y > ls) { L.wm()[k] = x.y | kNoStat; L.wx()[k] = 2 + s2; ++k; }
-                    if (x.z > ls) { L.wm()[k] = x.z | kNoStat; L.wx()[k] = 2 + s2; ++k; }
+                    if (x.x > ls) { v_wm[k] = x.x | kNoStat; v_wx[k] = 2 + s2; ++k; }
+                    if (x.y > ls) { v_wm[k] = x.y | kNoStat; v_wx[k] = 2 + s2; ++k; }
+                    if (x.z > ls) { v_wm[k] = x.z | kNoStat; v_wx[k] = 2 + s2; ++k; }
                 }
             }
         }
@@ -1492,7 +1503,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             for (int base = 0; base < npairs; base += kW) {
                 const int q = base + lane;
                 if (q < npairs) {
-                    const int m = (base == 0 && q < ncand ? cent : L.wm()[q]) & 0xFFFFFF;
+                    const int m = (base == 0 && q < ncand ? cent : v_wm[q]) & 0xFFFFFF;
                     if (mark_get(S.mark + m) & lowS) bad = true;
                 }
             }
@@ -1504,7 +1515,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             // read rows that were not checked
             const bool need_full = first && !is_i &&
                                    ((lf0.nptnr == i && cI > 0) || (lf0.nptnr == j && cJ > 0) || cI >= 2 || cJ >= 2);
-            if (need_full && !(spec && L.spec()[myslot].off >= 0)) bad = true;
+            if (need_full && !(spec && v_spec[myslot].off >= 0)) bad = true;
             if (__ballot(bad) && lane == 0) atomicMin(&B.cut, bk);
         }
         bar_to(nb, 2);
@@ -1525,17 +1536,17 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const bool wantI = first && cI > 0, wantJ = first && cJ > 0;
     const unsigned long long mI = __ballot(wantI), mJ = __ballot(wantJ);
     const int nI = __popcll(mI), nsing = nI + __popcll(mJ);
-    const bool sing_fit = npairs + nsing <= L.maxw;
+    const bool sing_fit = npairs + nsing <= maxw;
     int sqI = -1, sqJ = -1;
     if (sing_fit) {
         const unsigned long long below = (1ull << lane) - 1ull;
         if (wantI) {
             sqI = npairs + __popcll(mI & below);
-            L.wm()[sqI] = lane;
+            v_wm[sqI] = lane;
         }
         if (wantJ) {
             sqJ = npairs + nI + __popcll(mJ & below);
-            L.wm()[sqJ] = lane | 0x100;
+            v_wm[sqJ] = lane | 0x100;
         }
     }
     const int ntot = sing_fit ? npairs + nsing : npairs;
@@ -1547,15 +1558,15 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         const int q = base + lane;
         if (q < ntot) {
             const bool single = q >= npairs;
-            const int ent = base == 0 && q < ncand ? cent : L.wm()[q];
-            const int owner = single ? 0 : L.wx()[q];
+            const int ent = base == 0 && q < ncand ? cent : v_wm[q];
+            const int owner = single ? 0 : v_wx[q];
             int m, stat, xc;
             Rec ra, rb;
             int4 oc;
             double2 bwa;
             if (single) {
                 // eventredo_down.f: ev_code(l, p) with l's record first
-                const DownInfo& di = L.down()[ent & 0xFF];
+                const DownInfo& di = v_down[ent & 0xFF];
                 const bool oj = (ent & 0x100) != 0;
                 m = oj ? j : i;
                 ra = di.r;
@@ -1576,7 +1587,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                     ra = P.rj;
                     bwa = P.bwj;
                 } else {
-                    const SpecInfo& si = L.spec()[owner - 2];
+                    const SpecInfo& si = v_spec[owner - 2];
                     xc = si.l;
                     ra = si.r;
                     bwa = si.bw;
@@ -1594,14 +1605,14 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             double tij = 1000000000.0;
             int ty = 0;
             predict(T, code, ra, rb, dyn_bptnr(ra.dyn) == m, bwa, tf, tij, ty);
-            L.wt()[q] = tij;
+            v_wt[q] = tij;
             if (single) {
-                L.wy()[q] = ty;  // checked where the op sequence uses it
+                v_wy[q] = ty;  // checked where the op sequence uses it
             } else {
                 if (ty == -99) status = kErrBadCode;
                 if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
-                L.wy()[q] = (((ty + 1) & 31) << 16) | (m & 0xFFFF);  // fields masked: position stays in bits 21..31
-                atomicMin(&L.slot()[owner].tkey, t_order(tij));
+                v_wy[q] = (((ty + 1) & 31) << 16) | (m & 0xFFFF);  // fields masked: position stays in bits 21..31
+                atomicMin(&v_slot[owner].tkey, t_order(tij));
             }
         }
     }
@@ -1609,8 +1620,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     for (int base = 0; base < npairs; base += kW) {
         const int q = base + lane;
         if (q < npairs) {
-            const int owner = L.wx()[q];
-            if (t_order(L.wt()[q]) == L.slot()[owner].tkey) atomicMin(&L.slot()[owner].packed, ((unsigned)q << 21) | (unsigned)L.wy()[q]);
+            const int owner = v_wx[q];
+            if (t_order(v_wt[q]) == v_slot[owner].tkey) atomicMin(&v_slot[owner].packed, ((unsigned)q << 21) | (unsigned)v_wy[q]);
         }
     }
     wave_sync();
@@ -1619,18 +1630,18 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     double ti = T0 + tf, tj = T0 + tf;
     int pi = -1, ci = -1, pj = -1, cj = -1;
     {
-        const unsigned a = L.slot()[0].packed;
+        const unsigned a = v_slot[0].packed;
         if (a != ~0u) {
-            const double bt = L.wt()[a >> 21];
+            const double bt = v_wt[a >> 21];
             if (bt < T0) {
                 ti = bt + tf;
                 pi = (int)(a & 0xFFFF);
                 ci = (int)((a >> 16) & 31) - 1;
             }
         }
-        const unsigned b = L.slot()[1].packed;
+        const unsigned b = v_slot[1].packed;
         if (j >= 0 && b != ~0u) {
-            const double bt = L.wt()[b >> 21];
+            const double bt = v_wt[b >> 21];
             if (bt < T0) {
                 tj = bt + tf;
                 pj = (int)(b & 0xFFFF);
@@ -1651,7 +1662,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     }
     // ---- the rest of each down bead's ops; full recomputes from the batch,
     //      others (not foreseen) cooperatively one at a time
-    const bool have_full = spec && L.spec()[myslot].off >= 0;
+    const bool have_full = spec && v_spec[myslot].off >= 0;
     for (int guard = 0; guard < 64 * 9; ++guard) {
         while (first && (rem > 0 || pending)) {
             if (pending) {
@@ -1660,12 +1671,12 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                     st_p = pi0;
                     st_c = ci0;
                 } else if (have_full) {
-                    const unsigned a = L.slot()[2 + myslot].packed;
+                    const unsigned a = v_slot[2 + myslot].packed;
                     st_t = T0 + tf;
                     st_p = -1;
                     st_c = -1;
                     if (a != ~0u) {
-                        const double bt = L.wt()[a >> 21];
+                        const double bt = v_wt[a >> 21];
                         if (bt < T0) {
                             st_t = bt + tf;
                             st_p = (int)(a & 0xFFFF);
@@ -1685,8 +1696,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                     pending = true;
                 else if (sq >= 0) {
                     // eventredo_down.f from the batch's prediction
-                    double tij = L.wt()[sq];
-                    const int ty = L.wy()[sq];
+                    double tij = v_wt[sq];
+                    const int ty = v_wy[sq];
                     if (ty == -99) status = kErrBadCode;
                     if (tij < 0.0 && fabs(tij) > 1.0e-10) status = kErrNegativeTime;
                     tij = tij + tf;
