@@ -837,8 +837,9 @@ __device__ inline void mark_clear(unsigned* p) {
 // A wavefront's place in a multi-wavefront round: -1 alone; 0 the round's
 // first event (always committed, writes as it goes); k >= 1 a speculative
 // event whose writes wait for the round's checks.  In a round every
-// wavefront passes the three round barriers once (footprints marked,
-// footprints checked, new calendar keys known), whichever way it leaves.
+// wavefront passes the kRoundBars round barriers once (footprints marked,
+// new calendar keys known), whichever way it leaves.
+constexpr int kRoundBars = 2;
 __device__ inline void bar_to(int& nb, int n) {
     while (nb < n) {
         __syncthreads();
@@ -847,10 +848,17 @@ __device__ inline void bar_to(int& nb, int n) {
 }
 __device__ inline int spec_abort(const Lds& L, int bk, int& nb) {
     if (lane_id() == 0) atomicMin(&L.batch()->cut, bk);
-    bar_to(nb, 3);
+    bar_to(nb, kRoundBars);
     return -1;
 }
 
+__device__ inline void round_clear_marks(const SysDev& S, int dent, int i, int j, int sp) {
+    const int lane = lane_id();
+    if (dent >= 0) mark_clear(S.mark + dent);
+    if (lane == 0) mark_clear(S.mark + i);
+    if (lane == 1 && j >= 0) mark_clear(S.mark + j);
+    if (sp >= 0) mark_clear(S.mark + sp);
+}
 // Events of a round committed: the prefix before the first cut-off event,
 // before the first event after an error, and before the first event whose
 // key is not below every new calendar key of the events before it (the
@@ -1351,7 +1359,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         reschedule_generic(S, T, L, c, P, extra_blk, xpulse_del);
         if (batch) {
             if (lane == 0) L.batch()->nbl[bk] = 0;
-            bar_to(nb, 3);
+            bar_to(nb, kRoundBars);
         }
         return;
     }
@@ -1479,6 +1487,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             }
         }
     }
+    int sp = -1;  // the wavefront's bead of the shoulder set, marked in a round
+    bool sp_w = false;
     // ---- a round's footprint check.  Marks per bead: bit k = wavefront k
     //      rewrites the bead's state, bit 8+k = it rewrites the bead's
     //      calendar entry.  Event k commits only if no earlier event of the
@@ -1489,11 +1499,12 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         PH_RESET(tph);
         Batch& B = *L.batch();
         const unsigned sbit = 1u << bk, lbit = 1u << (8 + bk);
-        const int sp = shoulder_bead(S, P, lane);
+        sp = shoulder_bead(S, P, lane);
         if (dent >= 0) mark_or(S.mark + dent, lbit);
         if (lane == 0) mark_or(S.mark + i, sbit | lbit);
         if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit);
-        const bool sp_w = (P.flags & kPairWroteSp) != 0, sp_r = (P.flags & kPairReadsSp) != 0;
+        sp_w = (P.flags & kPairWroteSp) != 0;
+        const bool sp_r = (P.flags & kPairReadsSp) != 0;
         if (bk == 0 && sp_w && sp >= 0) mark_or(S.mark + sp, sbit);
         bar_to(nb, 1);
         PH_END(S, kPhBMark, tph);
@@ -1518,15 +1529,12 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             if (need_full && !(spec && v_spec[myslot].off >= 0)) bad = true;
             if (__ballot(bad) && lane == 0) atomicMin(&B.cut, bk);
         }
-        bar_to(nb, 2);
         PH_END(S, kPhBCheck, tph);
-        // every wavefront has checked: clear this one's marks
-        if (dent >= 0) mark_clear(S.mark + dent);
-        if (lane == 0) mark_clear(S.mark + i);
-        if (lane == 1 && j >= 0) mark_clear(S.mark + j);
-        if (bk == 0 && sp_w && sp >= 0) mark_clear(S.mark + sp);
+        // the cut only moves down: a wavefront already past it stops here
+        // (the others learn theirs at the next barrier)
         if (bk >= B.cut) {
-            bar_to(nb, 3);
+            bar_to(nb, kRoundBars);
+            round_clear_marks(S, dent, i, j, bk == 0 && sp_w ? sp : -1);
             return;
         }
     }
@@ -1776,8 +1784,10 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             B.knew[bk] = kn;
             B.st[bk] = c.status;
         }
-        bar_to(nb, 3);
+        bar_to(nb, kRoundBars);
         PH_END(S, kPhBKeys, tph);
+        // every wavefront has checked: clear this one's marks
+        round_clear_marks(S, dent, i, j, bk == 0 && sp_w ? sp : -1);
         if (bk >= batch_accepted(B)) return;
         if (bk >= 1 && lane == 0) {
             gst(S.rec + (i), P.ri);
@@ -3101,7 +3111,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             pair_event(S, T, L, *L.sc(), key_idx(k), key_p(k), key_ct(k), wave);
         } else {
             int nb = 0;
-            bar_to(nb, 3);
+            bar_to(nb, kRoundBars);
         }
         PH_BEGIN(ttail);
         const int A = batch_accepted(B);
