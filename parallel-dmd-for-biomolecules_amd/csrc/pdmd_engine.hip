@@ -2942,8 +2942,13 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
     const Leaf* const gleaf = L.gleaf;
     int K = 0;
     unsigned pip = 0xFFFFFFFFu;  // lane m: index << 16 | partner of popped entry m
+    // one superblock (up to 1024 entries): pops read the block minima
+    // directly and an exclusion touches only its block (the round's rescans
+    // restore the superblock)
+    const bool one_sup = nsup == 1;
     while (true) {
-        const Key k = key_min_wave(lane < nsup ? sup[lane] : empty_key());
+        const Key k = one_sup ? key_min_wave(lane < nblk ? blk[lane] : empty_key())
+                              : key_min_wave(lane < nsup ? sup[lane] : empty_key());
         if (k.bc == kEmptyBc) {
             if (K == 0) c.status = kErrBin;
             break;
@@ -2966,6 +2971,7 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
         }
         if (key_min_rows(x) && lane < kBlk) blk[b] = x;
         wave_sync();
+        if (one_sup) continue;
         const int bb = sp * kSup + lane;
         const Key y = key_min_wave(bb < nblk ? blk[bb] : empty_key());
         if (lane == 0) sup[sp] = y;
