@@ -706,6 +706,42 @@ struct Lds {
 constexpr int kLdOff = kSdOff + (int)((sizeof(SysDev) + 15) / 16 * 16);
 constexpr int kLdStride = (int)((sizeof(Lds) + 15) / 16 * 16);
 __device__ inline const Lds& ldsv() { return *reinterpret_cast<const Lds*>(smem + kLdOff + wave_id() * kLdStride); }
+template <class T>
+__device__ inline T* sgpr_ptr(T* p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
+// The wavefront's views as a value in scalar registers, for functions with
+// many wave fences (after each one the compiler would read a view's offset
+// from LDS again before using it).
+__device__ inline Lds lds_regs() {
+    const Lds& m = ldsv();
+    Lds r;
+    r.o_blk = __builtin_amdgcn_readfirstlane(m.o_blk);
+    r.o_sup = __builtin_amdgcn_readfirstlane(m.o_sup);
+    r.o_wm = __builtin_amdgcn_readfirstlane(m.o_wm);
+    r.o_wx = __builtin_amdgcn_readfirstlane(m.o_wx);
+    r.o_blist = __builtin_amdgcn_readfirstlane(m.o_blist);
+    r.o_pair = __builtin_amdgcn_readfirstlane(m.o_pair);
+    r.o_sc = __builtin_amdgcn_readfirstlane(m.o_sc);
+    r.o_leaf = __builtin_amdgcn_readfirstlane(m.o_leaf);
+    r.o_wd = __builtin_amdgcn_readfirstlane(m.o_wd);
+    r.o_wt = __builtin_amdgcn_readfirstlane(m.o_wt);
+    r.o_wy = __builtin_amdgcn_readfirstlane(m.o_wy);
+    r.o_slot = __builtin_amdgcn_readfirstlane(m.o_slot);
+    r.o_spec = __builtin_amdgcn_readfirstlane(m.o_spec);
+    r.o_down = __builtin_amdgcn_readfirstlane(m.o_down);
+    r.o_msc = __builtin_amdgcn_readfirstlane(m.o_msc);
+    r.o_batch = __builtin_amdgcn_readfirstlane(m.o_batch);
+    r.gleaf = sgpr_ptr(m.gleaf);
+    r.nblk = __builtin_amdgcn_readfirstlane(m.nblk);
+    r.nsup = __builtin_amdgcn_readfirstlane(m.nsup);
+    r.maxw = __builtin_amdgcn_readfirstlane(m.maxw);
+    r.nw = __builtin_amdgcn_readfirstlane(m.nw);
+    r.leaf_lds = __builtin_amdgcn_readfirstlane(m.leaf_lds ? 1 : 0) != 0;
+    return r;
+}
 __device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : gld(L.gleaf + k); }
 __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
     if (L.leaf_lds)
@@ -717,7 +753,7 @@ __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
 // block minima of the listed blocks (4 blocks per pass, one row of 16 lanes
 // each); a block listed twice gets the same value twice
 __device__ void rescan_blocks(const Lds&, int N3, const int* list, int nb) {
-    const Lds& L = ldsv();
+    const Lds L = lds_regs();
 
     const int lane = lane_id(), k = lane & 15;
     // 16 blocks per step: the leaf loads of four passes are in flight together
@@ -740,7 +776,7 @@ __device__ void rescan_blocks(const Lds&, int N3, const int* list, int nb) {
 // superblock minima over the superblocks of the listed blocks (at most 64),
 // each reduced once
 __device__ void rescan_sups(const Lds&, const int* list, int nb) {
-    const Lds& L = ldsv();
+    const Lds L = lds_regs();
 
     const int lane = lane_id();
     unsigned long long touched = 0;
@@ -890,7 +926,7 @@ __device__ inline Rec load_rec(const SysDev&, const Pair& P, int m) {
 // l's up list and upper shoulder partners (partial_events.f:121-139).
 __device__ void full_one(const SysDev&, const Lds&, int l, const Rec& rl, double2 bwl, int4 xl, double tf,
                          double T0, double& tim, int& np, int& ct, int& status) {
-    const Lds& L = ldsv();
+    const Lds L = lds_regs();
 
     const SysDev& S = sdev();
 
@@ -1262,7 +1298,7 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
 // ghost entry), or -1
 __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, const Pair&, int extra_blk,
                            bool xpulse_del, int bk) {
-    const Lds& L = ldsv();
+    const Lds L = lds_regs();
 
     const SysDev& S = sdev();
 
@@ -2068,7 +2104,7 @@ __device__ inline bool terminal_t(int t) {
 // One pair event: outcome resolution (main.F90:1487-1634), dynamics, H-bond
 // and shoulder bookkeeping (main.F90:641-921), then partial_events.
 __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int i, int j, int ct, int bk) {
-    const Lds& L = ldsv();
+    const Lds L = lds_regs();
 
     const SysDev& S = sdev();
 
