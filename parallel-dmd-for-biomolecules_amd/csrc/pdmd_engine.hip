@@ -2804,11 +2804,14 @@ __device__ void output_event(const SysDev&, const Lds&, Sc&) {
     rescan_block_list(L, N + 3, 1);
 }
 
-// LDS plan: tables, SysDev, the views, the shared calendar tree, master
+// LDS plan: tables, SysDev, the views, the superblock minima, master
 // scalars and round record, then each wavefront's work area (wavefront 0's
 // is the larger: the list rebuild and the generic rescheduling use it as
-// scratch; its scalars are the master copy), then the leaves
-__device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap, int nw, int wave) {
+// scratch; its scalars are the master copy), then this system's block
+// minima, then as many calendar leaves as the workgroup's lds_bytes hold:
+// a small system in a launch sized for a large one keeps all its leaves in
+// LDS
+__device__ Lds make_lds(const SysDev&, int maxw, int lds_bytes, int nw, int wave) {
     const SysDev& S = sdev();
 
     Lds L;
@@ -2826,7 +2829,6 @@ __device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap, int
     take((int)sizeof(DevTables));        // the tables, at offset 0
     take((int)sizeof(SysDev));           // the system's SysDev, at kSdOff
     take(kLdStride * kMaxWaves);         // these views, at kLdOff
-    L.o_blk = take((int)sizeof(Key) * nblk_max);
     L.o_sup = take((int)sizeof(Key) * kSup);
     L.o_msc = take((int)sizeof(Sc));
     L.o_batch = take((int)sizeof(Batch));
@@ -2856,8 +2858,9 @@ __device__ Lds make_lds(const SysDev&, int nblk_max, int maxw, int leaf_cap, int
         take((int)sizeof(Pair));
         take((int)sizeof(Sc));
     }
+    L.o_blk = take((int)sizeof(Key) * L.nblk);
     L.o_leaf = p;
-    L.leaf_lds = N3 <= leaf_cap;
+    L.leaf_lds = N3 <= (lds_bytes - p) / (int)sizeof(Leaf);
     L.gleaf = S.leaf;
     return L;
 }
@@ -2918,11 +2921,11 @@ __device__ void copy_tables(const DevTables* src, DevTables* dst) {
 // ------------------------------------------------------------------------
 // Start-up of one system per wavefront: wrap, lists, pseudo-event seeds and
 // events() (main.F90:205-424 after the host-side H-bond restore).
-__global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, const int* ids, int nblk_max, int maxw) {
+__global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, const int* ids, int maxw) {
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
     const SysDev& S = sdev();
-    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, nblk_max, maxw, 0, 1, 0);
+    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, maxw, 0, 1, 0);
     __syncthreads();
     const Lds& L = ldsv();
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
@@ -3066,13 +3069,13 @@ __device__ inline void log_event(const SysDev& S, int& log_count, long long coll
 // evenly; flags & kStopAtOutput stops it right after its next output
 // pseudo-event (where the reference writes observables and configurations).
 __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc,
-                                                       int nblk_max, int maxw, int leaf_cap, long long budget, int flags) {
+                                                       int maxw, int lds_bytes, long long budget, int flags) {
     const int nw = (int)(blockDim.x >> 6), wave = wave_id(), lane = lane_id();
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
     const SysDev& S = sdev();
     if (lane == 0)
-        *reinterpret_cast<Lds*>(smem + kLdOff + wave * kLdStride) = make_lds(S, nblk_max, maxw, leaf_cap, nw, wave);
+        *reinterpret_cast<Lds*>(smem + kLdOff + wave * kLdStride) = make_lds(S, maxw, lds_bytes, nw, wave);
     __syncthreads();
     const Lds& L = ldsv();
     const int N = S.N, N3 = N + 3;
@@ -3502,7 +3505,7 @@ static int nblk_for(int n) { return (n + 3 + kBlk - 1) / kBlk; }
 
 hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxw, hipStream_t s) {
     const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0, 1);
-    hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW), lds, s, d_sys, d_ids, nblk_for(max_n), maxw);
+    hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW), lds, s, d_sys, d_ids, maxw);
     return hipGetLastError();
 }
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
@@ -3511,11 +3514,25 @@ hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_
     int nw = waves < 1 ? 1 : (waves > kMaxWaves ? kMaxWaves : waves);
     while (nw > 1 && engine_lds_bytes(nblk, maxw, 0, nw) > lds_limit) --nw;
     const size_t base = engine_lds_bytes(nblk, maxw, 0, nw);
-    int leaf_cap = 0;
-    if (lds_limit > base) leaf_cap = (int)((lds_limit - base) / sizeof(Leaf));
-    if (leaf_cap > max_n + 3) leaf_cap = max_n + 3;
-    const size_t lds = engine_lds_bytes(nblk, maxw, leaf_cap, nw);
-    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, d_inc, nblk, maxw, leaf_cap, budget_ticks, flags);
+    // up to one workgroup per compute unit: each takes the whole LDS
+    // budget, so every system keeps as many leaves in LDS as fit beside
+    // its own block minima; beyond that, the size the largest system needs,
+    // so that several workgroups share a compute unit
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            ncu = v > 0 ? v : 1;
+        else
+            ncu = 1;
+    }
+    size_t lds = lds_limit;
+    if (n > ncu) {
+        size_t want = base + sizeof(Leaf) * (size_t)(max_n + 3);
+        lds = want < lds_limit ? want : lds_limit;
+    }
+    if (lds < base) lds = base;
+    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, d_inc, maxw, (int)lds, budget_ticks, flags);
     return hipGetLastError();
 }
 int engine_max_waves(int max_n, int maxw, size_t lds_limit) {
