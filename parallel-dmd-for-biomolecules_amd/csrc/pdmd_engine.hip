@@ -2963,9 +2963,10 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
 // Pop up to kmax calendar entries in key order (wavefront 0): pair events
 // only after the first, since a pseudo-event runs alone.  Each popped entry
 // but the last is taken out of the min tree until the round's rescans.
-// Every speculative wavefront k gets a copy of the master scalars as they
-// stand when its event comes up in the serial order; the master takes the
-// first event.  Returns the number popped (0: stop, or an empty calendar).
+// Every speculative wavefront k gets a copy of the master scalars (which it
+// sets to its event's time and bin after the round's first barrier); the
+// master takes the first event.  Returns the number popped (0: stop, or an
+// empty calendar).
 __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
     const Lds& L = ldsv();
 
@@ -2981,13 +2982,15 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
     const Leaf* const gleaf = L.gleaf;
     int K = 0;
     unsigned pip = 0xFFFFFFFFu;  // lane m: index << 16 | partner of popped entry m
-    // one superblock (up to 1024 entries): pops read the block minima
-    // directly and an exclusion touches only its block (the round's rescans
-    // restore the superblock)
+    // one superblock (up to 1024 entries): the block minima stay in lane
+    // registers (lane b: block b) and an exclusion replaces one of them; the
+    // tree in LDS keeps its values, which the round's rescans of the popped
+    // blocks confirm or replace
     const bool one_sup = nsup == 1;
+    Key bmin = empty_key();
+    if (one_sup && lane < nblk) bmin = blk[lane];
     while (true) {
-        const Key k = one_sup ? key_min_wave(lane < nblk ? blk[lane] : empty_key())
-                              : key_min_wave(lane < nsup ? sup[lane] : empty_key());
+        const Key k = one_sup ? key_min_wave(bmin) : key_min_wave(lane < nsup ? sup[lane] : empty_key());
         if (k.bc == kEmptyBc) {
             if (K == 0) c.status = kErrBin;
             break;
@@ -3008,9 +3011,15 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
             for (int m = 0; m < K; ++m) popped |= (int)(__builtin_amdgcn_readlane((int)pip, m) >> 16) == idx;
             if (!popped) x = leaf_key(leaf_lds ? lleaf[idx] : gld(gleaf + idx), idx);
         }
-        if (key_min_rows(x) && lane < kBlk) blk[b] = x;
+        const bool win = key_min_rows(x);
+        if (one_sup) {
+            const unsigned long long wm = __ballot(win && lane < kBlk);
+            const Key nb = readlane_key(x, __ffsll((long long)wm) - 1);
+            if (lane == b) bmin = nb;
+            continue;
+        }
+        if (win && lane < kBlk) blk[b] = x;
         wave_sync();
-        if (one_sup) continue;
         const int bb = sp * kSup + lane;
         const Key y = key_min_wave(bb < nblk ? blk[bb] : empty_key());
         if (lane == 0) sup[sp] = y;
@@ -3023,16 +3032,6 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
         const Lds& Lk = *reinterpret_cast<const Lds*>(smem + kLdOff + k * kLdStride);
         if (lane < (int)(sizeof(Sc) / 8)) reinterpret_cast<long long*>(Lk.sc())[lane] = cw;
     }
-    wave_sync();
-    if (lane == 0)
-        for (int k = 1; k < K; ++k) {
-            Sc* ck = reinterpret_cast<const Lds*>(smem + kLdOff + k * kLdStride)->sc();
-            ck->old_tfalse = B.key[k - 1].t;
-            ck->tfalse = B.key[k].t;
-            ck->nbin = key_bin(B.key[k]);
-            ck->coll = c.coll + 1 + k;
-            ck->status = 0;
-        }
     wave_sync();
     const Key k0 = B.key[0];
     c.old_tfalse = c.tfalse;
@@ -3150,7 +3149,17 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             if (nw > 1) __syncthreads();
             continue;
         }
-        // a round of K >= 2 pair events, one per wavefront
+        // a round of K >= 2 pair events, one per wavefront; a speculative
+        // wavefront's scalars are the master's as its event comes up
+        if (wave >= 1 && wave < K && lane == 0) {
+            Sc& ck = *L.sc();
+            ck.old_tfalse = B.key[wave - 1].t;
+            ck.tfalse = B.key[wave].t;
+            ck.nbin = key_bin(B.key[wave]);
+            ck.coll = c.coll + wave;
+            ck.status = 0;
+        }
+        wave_sync();
         if (wave < K) {
             const Key k = B.key[wave];
             pair_event(S, T, L, *L.sc(), key_idx(k), key_p(k), key_ct(k), wave);

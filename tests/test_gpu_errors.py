@@ -1,0 +1,30 @@
+"""Device error paths, forced from the host: a list capacity too small for
+the system's neighbour lists (the rebuild's kErrListCapacity status comes back
+as an error of pdmd_system_add), and calls on ids that do not exist.  The
+context stays usable after each error, at 1 and 4 wavefronts."""
+import pytest
+
+import pdmd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("waves", [1, 4])
+def test_list_capacity_overflow_is_reported(waves):
+    sysd = pdmd.generate([16] * 8, density=0.01, tstar=0.20, seed=11)
+    eng = pdmd.Engine(waves=waves)
+    try:
+        with pytest.raises(RuntimeError, match="status 1"):
+            eng.add_system(sysd, seed=3, list_capacity=2)
+        sid = eng.add_system(sysd, seed=3)
+        eng.advance([sid], [3000])
+        st = eng.stats(sid)
+        assert st.status == 0 and st.coll == 3000
+        with pytest.raises(RuntimeError):
+            eng.advance([sid + 7], [10])
+        with pytest.raises(RuntimeError):
+            eng.advance([sid, sid], [10, 10])
+        eng.advance([sid], [1000])
+        assert eng.stats(sid).coll == 4000
+    finally:
+        eng.close()
