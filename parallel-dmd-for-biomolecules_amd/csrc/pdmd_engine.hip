@@ -3189,8 +3189,12 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             wave_sync();
         }
         if (wave < A) rescan_blocks(L, N3, L.blist(), B.nbl[wave]);
-        __syncthreads();
-        if (wave < A) rescan_sups(L, L.blist(), B.nbl[wave]);
+        // a single superblock is never read: selection pops from the block
+        // minima (its value is recomputed by the single-event paths)
+        if (L.nsup > 1) {
+            __syncthreads();
+            if (wave < A) rescan_sups(L, L.blist(), B.nbl[wave]);
+        }
         if (wave == 0) {
             // the master takes the committed events in order
             for (int k = 0; k < A; ++k) {
