@@ -610,6 +610,7 @@ struct Batch {
     int K;                // entries popped this round (0: stop)
     int cut;              // first wavefront whose event is not committed
     int rebuild;          // the round's sync event rebuilds the lists
+    int gbead;            // the bead of a ghost event leading the round
 };
 // Order proxy of a calendar key: the bin in the top 12 bits, then the top
 // 52 bits of the ordered time.  Proxies order keys as key_less does except
@@ -1813,6 +1814,10 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             if (j >= 0) {
                 const Key kb = leaf_key(lb, j);
                 if (key_less(kb, kn)) kn = kb;
+            } else {
+                // a ghost event leading the round: its next ghost time
+                const Key kg = leaf_key(leaf_get(L, S.N), S.N);
+                if (key_less(kg, kn)) kn = kg;
             }
         }
         kn = key_min_wave(kn);
@@ -2282,7 +2287,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
 }
 
 // Andersen ghost collision, main.F90:997-1049
-__device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
+__device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&, int bk) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
@@ -2353,9 +2358,13 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&) {
     if (status && !c.status) c.status = status;
     if (c.tfalse < c.old_tfalse) c.tfalse = c.old_tfalse;
     if (lane == 0) *L.pair() = P;
+    if (lane == 0 && bk >= 0) {
+        L.batch()->ty[bk] = 0;
+        L.batch()->gbead = i;
+    }
     wave_sync();
     PH_END(S, kPhGhost, tg0);
-    reschedule(S, T, L, c, *L.pair(), N / kBlk, false, -1);
+    reschedule(S, T, L, c, *L.pair(), N / kBlk, false, bk);
     return i;
 }
 
@@ -3000,7 +3009,9 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
         if (lane == 0) B.key[K] = k;
         if (lane == K) pip = k.ip;
         ++K;
-        if (e >= N || K >= kmax) break;
+        // a ghost event may lead a round (its draws come first in the
+        // serial order); sync and output events run alone
+        if (e > N || K >= kmax) break;
         // take e out of the tree: its block without the popped entries, then
         // its superblock
         const int b = e / kBlk, sp = b / kSup;
@@ -3139,7 +3150,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
                     ty = pair_event(S, T, L, c, e, jj, key_ct(k), -1);
                     if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
                 } else if (e == N) {
-                    jj = ghost_event(S, T, L, c);
+                    jj = ghost_event(S, T, L, c, -1);
                 } else {
                     output_event(S, L, c);
                     stop_out = (flags & kStopAtOutput) != 0;
@@ -3162,7 +3173,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
         wave_sync();
         if (wave < K) {
             const Key k = B.key[wave];
-            pair_event(S, T, L, *L.sc(), key_idx(k), key_p(k), key_ct(k), wave);
+            if (key_idx(k) == N)
+                ghost_event(S, T, L, c, 0);
+            else
+                pair_event(S, T, L, *L.sc(), key_idx(k), key_p(k), key_ct(k), wave);
         } else {
             int nb = 0;
             bar_to(nb, kRoundBars);
@@ -3201,7 +3215,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
                 const Key kk = B.key[k];
                 const int ty = B.ty[k];
                 if (ty >= 1 && ty <= 30 && lane == 0) c.nev[ty]++;
-                log_event(S, log_count, c.coll + k, key_idx(kk), key_p(kk), ty, kk.t);
+                log_event(S, log_count, c.coll + k, key_idx(kk), key_idx(kk) == N ? B.gbead : key_p(kk), ty, kk.t);
             }
             if (A >= 2) {
                 if (lane == 0) {
