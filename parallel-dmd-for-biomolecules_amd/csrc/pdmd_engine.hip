@@ -915,6 +915,27 @@ __device__ inline int batch_accepted(const Batch& B) {
 // ------------------------------------------------------------------------
 // rescheduling after an event: partial_events.f:16-201
 // ------------------------------------------------------------------------
+// The list rows of i and j (lane k: slot k) with their lengths; they do not
+// change between list rebuilds, so a pair event loads them before it
+// resolves its outcome
+struct Rows {
+    int eui, edi, euj, edj;
+    int nui, ndi, nuj, ndj;
+};
+__device__ inline Rows load_rows(const SysDev& S, int i, int j) {
+    const int lane = lane_id(), cap = S.cap;
+    const int jr = j >= 0 ? j : i;
+    Rows r;
+    r.nui = gl1(S.nup + (i));
+    r.ndi = gl1(S.ndn + (i));
+    r.nuj = j >= 0 ? gl1(S.nup + (j)) : 0;
+    r.ndj = j >= 0 ? gl1(S.ndn + (j)) : 0;
+    r.eui = lane < cap ? gl1(S.nb_up + ((size_t)i * cap + lane)) : 0;
+    r.edi = lane < cap ? gl1(S.nb_dn + ((size_t)i * cap + lane)) : 0;
+    r.euj = lane < cap ? gl1(S.nb_up + ((size_t)jr * cap + lane)) : 0;
+    r.edj = lane < cap ? gl1(S.nb_dn + ((size_t)jr * cap + lane)) : 0;
+    return r;
+}
 __device__ inline Rec load_rec(const SysDev&, const Pair& P, int m) {
     const SysDev& S = sdev();
 
@@ -1298,7 +1319,7 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
 // the generic path.  extra_blk: a further calendar block to rescan (the
 // ghost entry), or -1
 __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, const Pair&, int extra_blk,
-                           bool xpulse_del, int bk) {
+                           bool xpulse_del, int bk, const Rows& rw) {
     const Lds L = lds_regs();
 
     const SysDev& S = sdev();
@@ -1327,13 +1348,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     int status = 0, nb = 0;
     PH_BEGIN(tph);
     // ---- round 1: list rows of i and j, whole rows with their lengths
-    const int jr = j >= 0 ? j : i;
-    const int nui = gl1(S.nup + (i)), ndi = gl1(S.ndn + (i));
-    const int nuj = j >= 0 ? gl1(S.nup + (j)) : 0, ndj = j >= 0 ? gl1(S.ndn + (j)) : 0;
-    const int eui = lane < cap ? gl1(S.nb_up + ((size_t)i * cap + lane)) : 0;
-    const int edi = lane < cap ? gl1(S.nb_dn + ((size_t)i * cap + lane)) : 0;
-    const int euj = lane < cap ? gl1(S.nb_up + ((size_t)jr * cap + lane)) : 0;
-    const int edj = lane < cap ? gl1(S.nb_dn + ((size_t)jr * cap + lane)) : 0;
+    const int nui = rw.nui, ndi = rw.ndi, nuj = rw.nuj, ndj = rw.ndj;
+    const int eui = rw.eui, edi = rw.edi, euj = rw.euj, edj = rw.edj;
     const int xi0 = P.xi.x, xi1 = P.xi.y, xi2 = P.xi.z;
     const int xj0 = P.xj.x, xj1 = P.xj.y, xj2 = P.xj.z;
     const int kNoStat = 0xFF << 24;
@@ -2131,6 +2147,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     P.xi = gld(S.extra + (i));
     P.xj = gld(S.extra + (j));
     const int4 oi = gld(S.ovr + (i));
+    const Rows rows = load_rows(S, i, j);  // in flight while the outcome is resolved
     const int evcode = dyn_code(oi, j, static_t(P.ri.topo, P.rj.topo));  // ev_code(i, j)
     const bool ok = !terminal_t(P.ri.topo) && !terminal_t(P.rj.topo);
     const int idi = dyn_ident(P.ri.dyn), idj = dyn_ident(P.rj.dyn);
@@ -2282,7 +2299,7 @@ __device__ int pair_event(const SysDev&, const DevTables&, const Lds&, Sc&, int 
     if (lane == 0) *L.pair() = P;
     wave_sync();
     PH_END(S, kPhPair, tpe);
-    reschedule(S, T, L, c, *L.pair(), -1, xpulse, bk);
+    reschedule(S, T, L, c, *L.pair(), -1, xpulse, bk, rows);
     return ct;
 }
 
@@ -2364,7 +2381,7 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&, int
     }
     wave_sync();
     PH_END(S, kPhGhost, tg0);
-    reschedule(S, T, L, c, *L.pair(), N / kBlk, false, bk);
+    reschedule(S, T, L, c, *L.pair(), N / kBlk, false, bk, load_rows(S, i, -1));
     return i;
 }
 
