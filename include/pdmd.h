@@ -31,6 +31,7 @@ typedef struct pdmd_ctx pdmd_ctx;
  * project RNG that replaces IFPORT drandm; canon != 0 enables Andersen
  * ghost collisions (-Dcanon). */
 #define PDMD_NO_HBS 1 /* no N-C hydrogen-bond wells: the reference's code 15 pairs stay hard cores */
+#define PDMD_NO_H 2   /* build flag -Dno_h (inputinfo.f:270-280): no code-16 wells between chains; same-chain ones stay */
 
 typedef struct {
     int32_t nchains;
@@ -43,7 +44,7 @@ typedef struct {
     uint64_t seed;
     int32_t canon;
     int32_t list_capacity; /* per-bead up/down list slots, at most 64; 0 = default (64) */
-    int32_t flags;         /* PDMD_NO_HBS: build flag -Dno_hbs (make_code.f:114-170,192-208,348-365) */
+    int32_t flags;         /* PDMD_NO_HBS (-Dno_hbs, make_code.f:114-170,192-208,348-365) | PDMD_NO_H (-Dno_h) */
     int32_t pad;
     const double* sv;
     const int32_t* bptnr;

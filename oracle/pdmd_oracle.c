@@ -106,6 +106,7 @@ struct orc_sys {
     double setemp, avegtime, sig_max_all, hdelr, rlsq[51];
     int canon;
     int no_hbs; /* -Dno_hbs: no N-C hydrogen-bond wells (make_code.f:114-170,192-208,348-365) */
+    int no_h;   /* -Dno_h: hp zeroed where make_code.f:80-104 reads it (inputinfo.f:270-280) */
     int64_t coll, numghosts, nupdates, nforced, nevents[31];
     uint64_t seed, rng_ctr[2];
     double old_tfalse;
@@ -362,7 +363,7 @@ static void make_code(orc_sys* S) {
     /* all r's: 16 for hp side chains on different chains (:74-104) */
     for (int m = 1; m <= N - 1; ++m)
         for (int l = m + 1; l <= N; ++l)
-            if (id[m] > 8 && id[l] > 8 && S->hp_bead[m] == 1 && S->hp_bead[l] == 1 &&
+            if (!S->no_h && id[m] > 8 && id[l] > 8 && S->hp_bead[m] == 1 && S->hp_bead[l] == 1 &&
                 S->chnnum[l] != S->chnnum[m])
                 EV(S, m, l) = 16;
     if (!S->no_hbs) {
@@ -1874,6 +1875,7 @@ orc_sys* orc_create_ex(const char* param_dir, int nchains, const int32_t* chain_
     S->N = N;
     S->canon = canon;
     S->no_hbs = (flags & ORC_NO_HBS) != 0;
+    S->no_h = (flags & ORC_NO_H) != 0;
     S->seed = seed;
     size_t n1 = (size_t)N + 4;
     S->chnnum = (int*)calloc(n1, sizeof(int));

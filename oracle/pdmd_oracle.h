@@ -58,6 +58,10 @@ orc_sys* orc_create(const char* param_dir, int nchains, const int32_t* chain_len
 /* The same with build flags: ORC_NO_HBS restates -Dno_hbs (no N-C
  * hydrogen-bond wells, make_code.f:114-170,192-208,348-365). */
 #define ORC_NO_HBS 1
+/* ORC_NO_H restates -Dno_h: hp zeroed for the side chains (inputinfo.f:270-280),
+ * which removes the code-16 wells between chains (make_code.f:80-104); the
+ * same-chain ones read hp1/hp2 and stay. */
+#define ORC_NO_H 2
 orc_sys* orc_create_ex(const char* param_dir, int nchains, const int32_t* chain_len,
                        const int32_t* species, const int32_t* restype, const int32_t* hp,
                        double boxl_angstrom, double tstar, uint64_t seed, int canon, int flags,

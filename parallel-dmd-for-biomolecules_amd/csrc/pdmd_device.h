@@ -33,15 +33,17 @@ struct __attribute__((aligned(16))) Leaf {
     int meta;  // (bin + 1) in the low 16 bits, coltype in the high 16 (signed)
 };
 
-__host__ __device__ inline int topo_chain(int t) { return t & 0x1FFF; }
+__host__ __device__ inline int topo_chain(int t) { return t & 0xFFF; }
+// -Dno_h: side chains of different chains never form code-16 wells
+__host__ __device__ inline int topo_noh(int t) { return (t >> 12) & 1; }
 __host__ __device__ inline int topo_kind(int t) { return (t >> 13) & 3; }
 __host__ __device__ inline int topo_res(int t) { return ((t >> 15) & 127) + 1; }
 __host__ __device__ inline int topo_len(int t) { return ((t >> 22) & 127) + 1; }
 __host__ __device__ inline int topo_hp(int t) { return (t >> 29) & 1; }
 __host__ __device__ inline int topo_pro(int t) { return (t >> 30) & 1; }
 __host__ __device__ inline int topo_nohbs(int t) { return t < 0 ? 1 : 0; }
-__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro, int nohbs) {
-    return (int)((unsigned)((chain & 0x1FFF) | (kind << 13) | ((r - 1) << 15) | ((L - 1) << 22) | (hp << 29) |
+__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro, int nohbs, int noh) {
+    return (int)((unsigned)((chain & 0xFFF) | (noh << 12) | (kind << 13) | ((r - 1) << 15) | ((L - 1) << 22) | (hp << 29) |
                             (pro << 30)) |
                  ((unsigned)nohbs << 31));
 }

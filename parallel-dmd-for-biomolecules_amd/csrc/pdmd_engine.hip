@@ -224,7 +224,9 @@ __device__ inline int static_t0(int ta, int tb) {
             default: return 1;
         }
     }
-    if (ka == 3 && kb == 3) return (topo_hp(ta) && topo_hp(tb)) ? 16 : 1;
+    // -Dno_h zeroes hp only where make_code.f:80-104 reads it (different
+    // chains); the same-chain loops read hp1/hp2 (make_code.f:174-186)
+    if (ka == 3 && kb == 3) return (topo_hp(ta) && topo_hp(tb) && !topo_noh(ta)) ? 16 : 1;
     if (ka == 1 && kb == 2) return (topo_pro(ta) && topo_chain(ta) < topo_chain(tb)) ? 1 : 15;
     if (ka == 2 && kb == 1) return (topo_pro(tb) && topo_chain(tb) < topo_chain(ta)) ? 1 : 15;
     return 1;

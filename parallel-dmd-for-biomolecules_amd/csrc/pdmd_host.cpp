@@ -261,7 +261,9 @@ int static_code(const Topology& t, int a, int b) {
         }
     }
     // different chains
-    if (ka == 3 && kb == 3) return (t.hp[t.res[a]] == 1 && t.hp[t.res[b]] == 1) ? 16 : 1;
+    // -Dno_h zeroes hp for make_code.f:80-104 only (inputinfo.f:270-280);
+    // the same-chain 16s above come from hp1/hp2, which it leaves alone
+    if (ka == 3 && kb == 3) return (!t.no_h && t.hp[t.res[a]] == 1 && t.hp[t.res[b]] == 1) ? 16 : 1;
     if ((ka == 1 && kb == 2) || (ka == 2 && kb == 1)) {
         if (t.no_hbs) return 1;  // make_code.f:114-170 skipped under -Dno_hbs
         // proline NH rule (make_code.f:139-161): kept only where the Pro side
@@ -328,9 +330,9 @@ void nbor_setup(const Params& p, const Topology& t, const std::vector<int>& iden
             for (const Cls& x : cls[qa])
                 for (const Cls& y : cls[qb]) {
                     int code = 1;
-                    if (x.kind == 3 && y.kind == 3) code = (x.hp == 1 && y.hp == 1) ? 16 : 1;
-                    else if (x.kind == 1 && y.kind == 2) code = x.pro ? 1 : 15;  // N in the earlier chain
-                    else if (x.kind == 2 && y.kind == 1) code = 15;
+                    if (x.kind == 3 && y.kind == 3) code = (!t.no_h && x.hp == 1 && y.hp == 1) ? 16 : 1;
+                    else if (x.kind == 1 && y.kind == 2) code = x.pro || t.no_hbs ? 1 : 15;  // N in the earlier chain
+                    else if (x.kind == 2 && y.kind == 1) code = t.no_hbs ? 1 : 15;
                     const int a = x.ident, b = y.ident;
                     if (code > 26) continue;
                     double s;

@@ -44,6 +44,7 @@ struct Topology {
     std::vector<int> restype, hp;                        // per residue (side-chain identity 9..28, hp flag)
     std::vector<int> chain, loc, res, ident0;            // per bead (0-based chain/res; loc 1-based)
     bool no_hbs = false;                                 // -Dno_hbs: no code-15 N-C wells
+    bool no_h = false;                                   // -Dno_h: no inter-chain code-16 wells
     void build(const std::vector<int>& chain_len, const std::vector<int>& species,
                const std::vector<int>& restype, const std::vector<int>& hp);
 };

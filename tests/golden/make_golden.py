@@ -203,10 +203,12 @@ def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000):
 def main():
     exe = os.path.join(OUT, "ref_harness")
     exe_n = os.path.join(OUT, "ref_harness_nohbs")
-    if not (os.path.exists(exe) and os.path.exists(exe_n)):
+    exe_h = os.path.join(OUT, "ref_harness_noh")
+    if not all(os.path.exists(e) for e in (exe, exe_n, exe_h)):
         raise SystemExit("build the checker first: make -C oracle/ref")
     run_case("ref_2species", exe, seed=4242)
     run_case("ref_2species_nohbs", exe_n, seed=4243)
+    run_case("ref_2species_noh", exe_h, seed=4244)
 
 
 if __name__ == "__main__":

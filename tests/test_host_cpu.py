@@ -36,16 +36,19 @@ def test_library_exports_every_declared_symbol():
     assert b"gfx950" in L.pdmd_version()
 
 
-@pytest.mark.parametrize("name,no_hbs", [("ref_2species", False), ("ref_2species_nohbs", True)])
-def test_pair_codes_match_reference_make_code(name, no_hbs):
+@pytest.mark.parametrize("name,flags", [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}),
+                                        ("ref_2species_noh", {"no_h": True})])
+def test_pair_codes_match_reference_make_code(name, flags):
     fx = dict(np.load(os.path.join(ROOT, "tests", "golden", name + ".npz")))
     sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"], boxl=float(fx["boxl"]),
                 tstar=float(fx["tstar"]))
     N = fx["identity"].size
     a, b = np.meshgrid(np.arange(1, N + 1), np.arange(1, N + 1), indexing="ij")
-    got = pdmd.pair_codes(sysd, a.ravel(), b.ravel(), species=fx["species"], no_hbs=no_hbs).reshape(N, N)
+    got = pdmd.pair_codes(sysd, a.ravel(), b.ravel(), species=fx["species"], **flags).reshape(N, N)
     ref = fx["ev_code"]
-    assert (15 in ref) != no_hbs
+    same = fx["chnnum"][:, None] == fx["chnnum"][None, :]
+    assert (15 in ref) != bool(flags.get("no_hbs"))
+    assert ((ref == 16) & ~same).any() != bool(flags.get("no_h"))
     off = ~np.eye(N, dtype=bool)
     bad = np.argwhere((got != ref) & off)
     assert bad.size == 0, f"first differing pair {bad[0] + 1}: {got[tuple(bad[0])]} vs {ref[tuple(bad[0])]}"

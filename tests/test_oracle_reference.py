@@ -3,7 +3,8 @@
 The fixtures in tests/golden/ come from oracle/_ref/ref_harness (the
 reference's unmodified Fortran compiled by oracle/ref/Makefile, driven by
 tests/golden/make_golden.py) on a two-species system (2 chains x 8 residues
-and 2 chains x 6 residues, rho* = 0.01, T* = 0.20): the scaled parameter
+and 2 chains x 6 residues, rho* = 0.01, T* = 0.20), built plain, with
+-Dno_hbs and with -Dno_h: the scaled parameter
 tables (inputinfo.f + scale_down.f), ev_code (make_code.f), the neighbour
 lists (nbor_setup.f, cell_link.f, nbor.f), the calendar after events.f,
 every pair prediction (core/bond/sqwel/sqshlder/nc_sqwel.f), the energy and
@@ -20,17 +21,18 @@ import pytest
 from oracle import Oracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CASES = [("ref_2species", False), ("ref_2species_nohbs", True)]
+# (fixture, build flags of the reference harness that wrote it)
+CASES = [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}), ("ref_2species_noh", {"no_h": True})]
 
 
 def _load(name):
     return dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
 
 
-def _oracle(fx, param_dir, no_hbs):
+def _oracle(fx, param_dir, flags):
     sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"],
                 boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
-    return Oracle(sysd, seed=1, param_dir=param_dir, canon=False, species=fx["species"], no_hbs=no_hbs)
+    return Oracle(sysd, seed=1, param_dir=param_dir, canon=False, species=fx["species"], **flags)
 
 
 def _bits(a):
@@ -39,9 +41,9 @@ def _bits(a):
 
 @pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
 def case(request, param_dir):
-    name, no_hbs = request.param
+    name, flags = request.param
     fx = _load(name)
-    return fx, _oracle(fx, param_dir, no_hbs), no_hbs
+    return fx, _oracle(fx, param_dir, flags), flags
 
 
 def test_tables(case):
@@ -66,13 +68,18 @@ def test_tables(case):
 
 
 def test_ev_code(case):
-    fx, orc, no_hbs = case
+    fx, orc, flags = case
     N = orc.N
     got = np.array([[orc.ev_code(a, b) for b in range(1, N + 1)] for a in range(1, N + 1)], np.int32)
     ref = fx["ev_code"]
     bad = np.argwhere(got != ref)
     assert bad.size == 0, f"{len(bad)} codes differ, first (a,b)={bad[0] + 1}: oracle {got[tuple(bad[0])]} ref {ref[tuple(bad[0])]}"
-    assert (15 in ref) != no_hbs
+    # what each build flag removes: -Dno_hbs every code 15, -Dno_h the code
+    # 16 wells between chains (the same-chain ones read hp1/hp2 and stay)
+    same = fx["chnnum"][:, None] == fx["chnnum"][None, :]
+    assert (15 in ref) != bool(flags.get("no_hbs"))
+    assert ((ref == 16) & ~same).any() != bool(flags.get("no_h"))
+    assert ((ref == 16) & same).any()
 
 
 def test_neighbour_lists_in_reference_order(case):
@@ -121,9 +128,9 @@ def test_observables_initial(case):
 
 def test_serial_loop(param_dir):
     # fresh oracles: the module fixture's ones have been queried only
-    for name, no_hbs in CASES:
+    for name, flags in CASES:
         fx = _load(name)
-        orc = _oracle(fx, param_dir, no_hbs)
+        orc = _oracle(fx, param_dir, flags)
         n = int(fx["loop_n"])
         keep = fx["loop_i"].size
         log = orc.run(n, log_cap=keep)
