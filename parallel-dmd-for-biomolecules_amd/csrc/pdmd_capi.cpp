@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/pdmd.h"
@@ -299,49 +300,84 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
             if (b < 0 || b > N) return fail("pdmd_system_add: bptnr out of range");
             bptnr[k] = b - 1;
         }
-    // -Drunr restore of H-bond identities and shoulders, main.F90:240-321:
-    // pair loop k < kj restricted to the pairs either branch can act on
+    // -Drunr restore of H-bond identities and shoulders, main.F90:240-321.
+    // The reference walks every pair k < kj in order, but only two kinds of
+    // pair act: an N-C pair (identities 1 + 4) inside its well, and
+    // (k, bptnr(k)).  The N-C pairs closer than the wider of the two N-C
+    // wells come from a cell grid; sorted into (k, kj) order with the
+    // bptnr pairs, they are visited in the reference's order, so identity
+    // changes made along the way reach the later pairs as they do there.
     HostCodes hc{t, ovr};
-    std::vector<int> nbeads, cbeads;
-    for (int k = 0; k < N; ++k) {
-        const int L = t.chain_len[t.chain[k]], kind = (t.loc[k] - 1) / L;
-        if (kind == 1) nbeads.push_back(k);
-        if (kind == 2) cbeads.push_back(k);
-    }
-    for (int k = 0; k < N; ++k) {
-        const int L = t.chain_len[t.chain[k]], kind = (t.loc[k] - 1) / L;
-        std::vector<int> cand;
-        if (kind == 1 || kind == 2) {
-            const std::vector<int>& other = kind == 1 ? cbeads : nbeads;
-            for (int q : other)
-                if (q > k) cand.push_back(q);
+    std::vector<std::pair<int, int>> cand;
+    {
+        const double w2 = std::max(P.welldia_sq[1][4], P.welldia_sq[4][1]);
+        auto r2_of = [&](int a, int b) {
+            double rx = bead[a].x - bead[b].x, ry = bead[a].y - bead[b].y, rz = bead[a].z - bead[b].z;
+            rx = rx - std::round(rx);
+            ry = ry - std::round(ry);
+            rz = rz - std::round(rz);
+            return rx * rx + ry * ry + rz * rz;
+        };
+        std::vector<int> nbead, cbead;
+        for (int k = 0; k < N; ++k) {
+            const int L = t.chain_len[t.chain[k]], kind = (t.loc[k] - 1) / L;
+            if (kind == 1) nbead.push_back(k);
+            if (kind == 2) cbead.push_back(k);
         }
-        if (bptnr[k] > k && (cand.empty() || std::find(cand.begin(), cand.end(), bptnr[k]) == cand.end())) {
-            cand.push_back(bptnr[k]);
-            std::sort(cand.begin(), cand.end());
-        }
-        for (int kj : cand) {
-            if (ident[k] + ident[kj] == 5 && hc.dyn(k, kj) == 15) {
-                double rx = bead[k].x - bead[kj].x, ry = bead[k].y - bead[kj].y, rz = bead[k].z - bead[kj].z;
-                rx = rx - std::round(rx);
-                ry = ry - std::round(ry);
-                rz = rz - std::round(rz);
-                const double r2 = rx * rx + ry * ry + rz * rz;
-                const double diff = r2 - P.welldia_sq[ident[k]][ident[kj]];
-                if (diff < 0.0 && !terminal_bead(t, k) && !terminal_bead(t, kj)) {
-                    const bool ok = ident[k] == 1 ? host_repuls_add(t, hc, extra, k, kj)
-                                                  : host_repuls_add(t, hc, extra, kj, k);
-                    if (!ok) return fail("pdmd_system_add: override slots exhausted in H-bond restore");
-                }
+        // cells at least 1 + 1e-6 well radii wide (a pair closer than the
+        // well sits in neighbouring cells; positions are wrapped to
+        // [-1/2, 1/2]), and about four beads per cell at most
+        const int nc = std::min((int)std::floor(1.0 / (std::sqrt(w2) * (1.0 + 1e-6))), (int)std::cbrt(4.0 * N) + 1);
+        if (nc >= 3) {
+            auto cell_of = [&](double x) { return std::min(nc - 1, std::max(0, (int)std::floor((x + 0.5) * nc))); };
+            std::vector<int> head((size_t)nc * nc * nc, -1), next(N, -1);
+            for (int q : cbead) {
+                const int c = cell_of(bead[q].x) + nc * (cell_of(bead[q].y) + nc * cell_of(bead[q].z));
+                next[q] = head[c];
+                head[c] = q;
             }
-            if (kj == bptnr[k]) {
-                if (ident[k] == 1) {
-                    ident[k] = 5;
-                    ident[kj] = 8;
-                } else {
-                    ident[k] = 8;
-                    ident[kj] = 5;
-                }
+            for (int k : nbead) {
+                const int cx = cell_of(bead[k].x), cy = cell_of(bead[k].y), cz = cell_of(bead[k].z);
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int c = (cx + dx + nc) % nc + nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc));
+                            for (int q = head[c]; q >= 0; q = next[q])
+                                if (r2_of(k, q) < w2) cand.emplace_back(std::min(k, q), std::max(k, q));
+                        }
+            }
+        } else {
+            for (int k : nbead)
+                for (int q : cbead)
+                    if (r2_of(k, q) < w2) cand.emplace_back(std::min(k, q), std::max(k, q));
+        }
+        for (int k = 0; k < N; ++k)
+            if (bptnr[k] > k) cand.emplace_back(k, bptnr[k]);
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    }
+    for (const auto& pr : cand) {
+        const int k = pr.first, kj = pr.second;
+        if (ident[k] + ident[kj] == 5 && hc.dyn(k, kj) == 15) {
+            double rx = bead[k].x - bead[kj].x, ry = bead[k].y - bead[kj].y, rz = bead[k].z - bead[kj].z;
+            rx = rx - std::round(rx);
+            ry = ry - std::round(ry);
+            rz = rz - std::round(rz);
+            const double r2 = rx * rx + ry * ry + rz * rz;
+            const double diff = r2 - P.welldia_sq[ident[k]][ident[kj]];
+            if (diff < 0.0 && !terminal_bead(t, k) && !terminal_bead(t, kj)) {
+                const bool ok = ident[k] == 1 ? host_repuls_add(t, hc, extra, k, kj)
+                                              : host_repuls_add(t, hc, extra, kj, k);
+                if (!ok) return fail("pdmd_system_add: override slots exhausted in H-bond restore");
+            }
+        }
+        if (kj == bptnr[k]) {
+            if (ident[k] == 1) {
+                ident[k] = 5;
+                ident[kj] = 8;
+            } else {
+                ident[k] = 8;
+                ident[kj] = 5;
             }
         }
     }
@@ -478,7 +514,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     if (ctx->d_ids.n < 1) HIPCHK(ctx->d_ids.alloc(4096));
     if (ctx->d_inc.n < 1) HIPCHK(ctx->d_inc.alloc(4096));
     HIPCHK(hipMemcpy(ctx->d_ids.p, &id, sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, engine_maxw(H.cap), ctx->stream));
+    HIPCHK(engine_setup(ctx->d_sys.p, ctx->d_ids.p, 1, N, engine_maxw(H.cap), ctx->lds_limit, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     SysState chk;
     HIPCHK(hipMemcpy(&chk, H.st.p, sizeof chk, hipMemcpyDeviceToHost));

@@ -15,8 +15,10 @@ int engine_max_waves(int max_n, int maxw, size_t lds_limit);
 int engine_maxw(int cap);
 // Opt the event-loop kernels in to lds_limit bytes of LDS per workgroup.
 hipError_t engine_prepare(size_t lds_limit);
-// Lists, pseudo-event seeds and events() for n systems (one wavefront each).
-hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, hipStream_t s);
+// Lists, pseudo-event seeds and events() for n systems (one workgroup of up
+// to 4 wavefronts each, as many as fit in lds_limit).
+hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, size_t lds_limit,
+                        hipStream_t s);
 // Event loop: every listed system runs until its SysState::target.
 // budget_ticks: wall_clock64 ticks after which each system stops between
 // rounds (0 = run to the event targets); waves: wavefronts per system (1 to

@@ -2949,11 +2949,16 @@ __device__ void copy_tables(const DevTables* src, DevTables* dst) {
 // ------------------------------------------------------------------------
 // Start-up of one system per wavefront: wrap, lists, pseudo-event seeds and
 // events() (main.F90:205-424 after the host-side H-bond restore).
-__global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, const int* ids, int maxw) {
+// One workgroup of up to kSetupWaves wavefronts per system: the list
+// rebuild and events() run on all of them, the rest on the first lane.
+constexpr int kSetupWaves = 4;
+__global__ __launch_bounds__(64 * kSetupWaves) void k_setup(const SysDev* __restrict__ sys, const int* ids, int maxw) {
+    const int nw = (int)(blockDim.x >> 6), wave = wave_id();
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
     const SysDev& S = sdev();
-    if (threadIdx.x == 0) *reinterpret_cast<Lds*>(smem + kLdOff) = make_lds(S, maxw, 0, 1, 0);
+    if (lane_id() == 0)
+        *reinterpret_cast<Lds*>(smem + kLdOff + wave * kLdStride) = make_lds(S, maxw, 0, nw, wave);
     __syncthreads();
     const Lds& L = ldsv();
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
@@ -2985,6 +2990,7 @@ __global__ __launch_bounds__(64) void k_setup(const SysDev* __restrict__ sys, co
     __threadfence_block();
     __syncthreads();
     rebuild(S, T, L, c);
+    __syncthreads();
     if (threadIdx.x == 0) store_sc(S.st, c);
 }
 
@@ -3549,9 +3555,12 @@ size_t engine_lds_bytes(int nblk_max, int maxw, int leaf_cap, int nw) {
 }
 static int nblk_for(int n) { return (n + 3 + kBlk - 1) / kBlk; }
 
-hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxw, hipStream_t s) {
-    const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0, 1);
-    hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW), lds, s, d_sys, d_ids, maxw);
+hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxw, size_t lds_limit,
+                        hipStream_t s) {
+    int nw = kSetupWaves;
+    while (nw > 1 && engine_lds_bytes(nblk_for(max_n), maxw, 0, nw) > lds_limit) --nw;
+    const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0, nw);
+    hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
     return hipGetLastError();
 }
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
