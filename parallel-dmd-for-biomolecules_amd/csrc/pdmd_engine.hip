@@ -2399,6 +2399,8 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&, int
 // cells, all cell ranges requested before the candidates, each candidate
 // read with two independent loads.  Candidates come in a fixed cell order
 // and ascending bead index within a cell, so the lists are deterministic.
+// v mod m for v in [-1, m]
+__device__ inline int wrap1(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
 __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     const Lds& L = ldsv();
 
@@ -2493,8 +2495,8 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         const int ii = act ? i : 0;
         const int fi = gl1(S.cfine + (ii));
         const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
-        const int k = gl1(S.ckey + (ii));
-        const int qx = k % Mq, qy = (k / Mq) % Mq, qz = k / (Mq * Mq);
+        // the coarse cell, as the binning step derived it from the fine one
+        const int qx = min(ix >> 1, Mq - 1), qy = min(iy >> 1, Mq - 1), qz = min(iz >> 1, Mq - 1);
         const Rec bi = gld(S.rec + (ii));
         const int4 oi = gld(S.ovr + (ii));
         int nr = 0;
@@ -2503,7 +2505,8 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
             const int a = t / 9, bb = (t / 3) % 3, cc = t % 3;
             if (act && a < noff && bb < noff && cc < noff) {
                 const int oa = a == 2 ? -1 : a, ob = bb == 2 ? -1 : bb, oc = cc == 2 ? -1 : cc;
-                const int nx = (qx + oa + Mq) % Mq, ny = (qy + ob + Mq) % Mq, nz = (qz + oc + Mq) % Mq;
+                // periodic neighbour of a coarse cell in [0, Mq) by one step
+                const int nx = wrap1(qx + oa, Mq), ny = wrap1(qy + ob, Mq), nz = wrap1(qz + oc, Mq);
                 const int cell = nx + Mq * (ny + Mq * nz);
                 const int c0 = gl1(S.cstart + (cell)), c1 = gl1(S.cstart + (cell + 1));
                 if (c1 > c0) {
@@ -2555,10 +2558,11 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
                 const int jb = inf[u].x;
                 if (jb == i) continue;
                 const int fj = inf[u].z;
+                // fine cells lie in [0, M): differences in (-M, M), taken mod M
                 int dx = (fj & 1023) - ix, dy = ((fj >> 10) & 1023) - iy, dz = (fj >> 20) - iz;
-                dx = ((dx % M) + M) % M;
-                dy = ((dy % M) + M) % M;
-                dz = ((dz % M) + M) % M;
+                dx += dx < 0 ? M : 0;
+                dy += dy < 0 ? M : 0;
+                dz += dz < 0 ? M : 0;
                 if ((dx > 2 && dx < M - 2) || (dy > 2 && dy < M - 2) || (dz > 2 && dz < M - 2)) continue;
                 const int stat = static_t(inf[u].y, bi.topo);
                 // ev_code(j,i), nbor.f:63: j's overrides exist only towards i's

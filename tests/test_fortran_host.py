@@ -6,7 +6,9 @@ hp*.inp, firstside*.data, helix.inp) for the same seeded system the Python
 binding builds; 'codes' mode reads them back as the reference driver would
 and passes a pdmd_system_desc through the C ABI (pair codes checked against
 the Python path).  GPU: 'run' mode commits entries on the device from those
-files and its .config/.lastvel records match the oracle bit for bit.
+files and its .config/.lastvel records match the oracle bit for bit, from
+the helix start and from a restart that reads the previous run's
+.config/.lastvel/.bptnr records (-Drunr).
 """
 import os
 import shutil
@@ -21,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "parallel-dmd-for-biomolecules_amd", "host")
 DRIVER = os.path.join(HOST, "pdmd_driver")
 NML = "&pdmd_run\n mode='{mode}', nchains1=3, chnln1=8, nchains2=2, chnln2=6, density=0.01, gen_seed=5,\n" \
-      " canon=1, seed=3, start='helix', waves={waves}, debugging=1 /\n"
+      " canon=1, seed=3, start='{start}', boxl={boxl}, waves={waves}, debugging=1 /\n"
 
 
 @pytest.fixture(scope="module")
@@ -31,9 +33,9 @@ def driver():
     return DRIVER
 
 
-def _run(driver, work, mode, tstar=0.20, ncoll=1000, waves=1):
+def _run(driver, work, mode, tstar=0.20, ncoll=1000, waves=1, start="helix", boxl=0.0):
     with open(os.path.join(work, "pdmd.nml"), "w") as f:
-        f.write(NML.format(mode=mode, waves=waves))
+        f.write(NML.format(mode=mode, waves=waves, start=start, boxl=repr(float(boxl))))
     r = subprocess.run([driver], input=f"{tstar}\n{ncoll}\n", cwd=work, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
@@ -143,3 +145,47 @@ def test_run_matches_oracle(driver, tmp_path, param_dir, waves):
     assert np.array_equal(np.frombuffer(bp[8:], np.int32), st["bptnr"])
     lines = open(os.path.join(work, "results", "run0000.energy")).read().splitlines()
     assert len(lines) >= 2 and int(lines[0].split()[0]) == 0
+
+
+def _dnint(x):
+    """Fortran dnint: nearest integer, halves away from zero."""
+    t = np.trunc(x)
+    return np.where(np.abs(x - t) >= 0.5, t + np.sign(x), t)
+
+
+@pytest.mark.gpu
+def test_restart_run_matches_oracle(driver, tmp_path, param_dir):
+    from oracle import Oracle
+
+    work = _workdir(tmp_path)
+    _run(driver, work, "generate")
+    _run(driver, work, "run", ncoll=3000)
+    N, B, _ = _read_helix(work)
+    # the restart reads run0000's last records (inputinfo.f:69-101) and writes run0001
+    ncoll = 2500
+    out = _run(driver, work, "run", ncoll=ncoll, start="restart", boxl=B)
+    assert "the final total energy" in out
+    rec = _records(os.path.join(work, "results", "run0000.config"))[-1]
+    xyz = np.frombuffer(rec[16:], np.float64).reshape(3, N)
+    v0 = np.frombuffer(_records(os.path.join(work, "results", "run0000.lastvel"))[-1][8:], np.float64).reshape(3, N)
+    bp0 = np.frombuffer(_records(os.path.join(work, "results", "run0000.bptnr"))[-1][8:], np.int32)
+    sv = np.empty((N, 6))
+    for d in range(3):
+        sv[:, d] = xyz[d] - _dnint(xyz[d])
+        sv[:, 3 + d] = v0[d]
+    sysd = dict(_python_system(), sv=sv.ravel(), boxl=B, tstar=0.20)
+    orc = Oracle(sysd, seed=3, param_dir=param_dir, canon=True, species=[0, 0, 0, 1, 1], bptnr=bp0)
+    orc.run(ncoll)
+    st, tf = orc.state(), orc.stats().tfalse
+    o = st["sv"].reshape(N, 6)
+    rec = _records(os.path.join(work, "results", "run0001.config"))[-1]
+    assert np.frombuffer(rec[:8], np.int64)[0] == ncoll
+    xyz = np.frombuffer(rec[16:], np.float64).reshape(3, N)
+    for d in range(3):
+        want = o[:, d] + o[:, 3 + d] * tf
+        assert np.array_equal(xyz[d].view(np.int64), want.view(np.int64))
+    v = np.frombuffer(_records(os.path.join(work, "results", "run0001.lastvel"))[-1][8:], np.float64).reshape(3, N)
+    for d in range(3):
+        assert np.array_equal(v[d].view(np.int64), o[:, 3 + d].view(np.int64))
+    bp = _records(os.path.join(work, "results", "run0001.bptnr"))[-1]
+    assert np.array_equal(np.frombuffer(bp[8:], np.int32), st["bptnr"])
