@@ -131,6 +131,15 @@ int pdmd_advance_to_output(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, 
  * whose master/worker speculation (main.F90:484-1258) this replaces.
  * Default 1. */
 int pdmd_set_waves(pdmd_ctx* ctx, int32_t waves_per_system);
+/* Launch mode of pdmd_advance*: with on != 0 the event-loop launch is
+ * captured in a HIP graph once per launch shape (systems count, largest
+ * system, budget, flags, wavefronts) and replayed while the shape stays the
+ * same; the system list and event targets travel in device buffers, so
+ * every step of a serving loop with an unchanged resident set is one graph
+ * replay.  Results do not depend on the mode.  No counterpart in the
+ * reference (its master drives MPI workers instead); the serving loop turns
+ * it on.  Default off. */
+int pdmd_set_graph_launch(pdmd_ctx* ctx, int32_t on);
 /* Autotuner of the wavefronts per system: advances the listed systems by
  * probe_events entries each at 1, 2, 4 and 8 wavefronts (those that fit in
  * LDS), one launch each, and keeps the fastest for later advances.  The

@@ -92,6 +92,19 @@ struct pdmd_ctx {
     int device = 0;
     long long wall_khz = 100000;  // wall_clock64 rate
     int waves = 1;                // wavefronts per system
+    // graph launches: the event-loop launch captured once per launch shape
+    // and replayed while the shape stays the same
+    bool graphs = false;
+    hipGraphExec_t gexec = nullptr;
+    struct Shape {
+        const void *sys, *ids, *inc;
+        int nsys, max_n, maxw, flags, waves;
+        long long ticks;
+        bool operator==(const Shape& o) const {
+            return sys == o.sys && ids == o.ids && inc == o.inc && nsys == o.nsys && max_n == o.max_n &&
+                   maxw == o.maxw && flags == o.flags && waves == o.waves && ticks == o.ticks;
+        }
+    } gshape{};
 };
 
 namespace pdmd {
@@ -220,6 +233,7 @@ int pdmd_ctx_destroy(pdmd_ctx* ctx) {
     }
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     delete ctx;
     return 0;
 }
@@ -594,8 +608,35 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     long long ticks = 0;
     if (budget_ms > 0.0) ticks = (long long)(budget_ms * (double)ctx->wall_khz);
-    HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit,
-                      ctx->stream, ticks, flags, ctx->waves));
+    const int maxw = engine_maxw(max_cap);
+    if (!ctx->graphs) {
+        HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, maxw, ctx->lds_limit, ctx->stream,
+                          ticks, flags, ctx->waves));
+    } else {
+        // the system list and event targets live in device buffers, so one
+        // captured launch serves every step of the same shape
+        const pdmd_ctx::Shape shape{ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, maxw, flags, ctx->waves,
+                                    ticks};
+        if (!ctx->gexec || !(shape == ctx->gshape)) {
+            if (ctx->gexec) {
+                HIPCHK(hipGraphExecDestroy(ctx->gexec));
+                ctx->gexec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            const hipError_t er = engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, maxw,
+                                             ctx->lds_limit, ctx->stream, ticks, flags, ctx->waves);
+            const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            const hipError_t ei = er == hipSuccess && ec == hipSuccess ? hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0)
+                                                                       : hipSuccess;
+            if (g) (void)hipGraphDestroy(g);
+            HIPCHK(er);
+            HIPCHK(ec);
+            HIPCHK(ei);
+            ctx->gshape = shape;
+        }
+        HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     // the copies above read pageable memory synchronously before returning
@@ -605,6 +646,12 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
 }  // namespace pdmd
 
 extern "C" {
+
+int pdmd_set_graph_launch(pdmd_ctx* ctx, int32_t on) {
+    if (!ctx) return fail("pdmd_set_graph_launch: null context");
+    ctx->graphs = on != 0;
+    return 0;
+}
 
 int pdmd_set_waves(pdmd_ctx* ctx, int32_t waves_per_system) {
     if (!ctx) return fail("pdmd_set_waves: null context");

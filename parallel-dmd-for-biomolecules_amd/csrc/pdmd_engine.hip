@@ -54,12 +54,13 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(th
 // footprint marks up to the first round barrier, 25 footprint checks up to
 // the second, 26 new keys up to the third, 27 the round's rescans and merge,
 // 28 rounds cut by a footprint conflict or an event that cannot speculate,
-// 29 rounds cut by the key order.
+// 29 rounds cut by the key order; round selection: 30 the pops, 31 the
+// scalar copies after them.
 enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
        kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
        kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20,
        kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23, kPhBMark = 24, kPhBCheck = 25, kPhBKeys = 26,
-       kPhBTail = 27, kPhNCutMark = 28, kPhNCutKey = 29 };
+       kPhBTail = 27, kPhNCutMark = 28, kPhNCutKey = 29, kPhSelPop = 30, kPhSelCopy = 31 };
 #ifdef PDMD_PHASE_TIMERS
 #define PH_BEGIN(v) long long v = clock64()
 #define PH_END(S, k, v)                                                                                  \
@@ -3007,6 +3008,9 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_setup(const SysDev* __rest
 // empty calendar).
 __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
     const Lds& L = ldsv();
+    const SysDev& S = sdev();
+    (void)S;
+    PH_BEGIN(tsp);
 
     Batch& B = *L.batch();
     const int lane = lane_id();
@@ -3065,6 +3069,7 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
         if (lane == 0) sup[sp] = y;
         wave_sync();
     }
+    PH_END(S, kPhSelPop, tsp);
     wave_sync();
     if (K == 0) return 0;
     const long long cw = lane < (int)(sizeof(Sc) / 8) ? reinterpret_cast<const long long*>(&c)[lane] : 0;
@@ -3078,6 +3083,7 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
     c.nbin = key_bin(k0);
     c.tfalse = k0.t;
     c.coll++;
+    PH_END(S, kPhSelCopy, tsp);
     return K;
 }
 

@@ -162,6 +162,9 @@ int pdmd_server_create(pdmd_ctx* ctx, int64_t mem_budget_bytes, int64_t quantum,
     s->slice_ms = slice_ms;
     s->max_resident = max_resident > 0 ? (max_resident < 4096 ? max_resident : 4096) : 4096;
     s->t0 = now_s();
+    // each step's launch is a replay of the captured event-loop launch
+    // while the resident set keeps its shape
+    if (int rc = pdmd_set_graph_launch(ctx, 1)) return rc;
     *out = s.release();
     return 0;
 }
@@ -171,6 +174,7 @@ int pdmd_server_destroy(pdmd_server* s) {
     if (s->inflight) pdmd_wait(s->ctx);
     for (Req* r : s->resident)
         if (r->sys >= 0) pdmd_system_remove(s->ctx, r->sys);
+    pdmd_set_graph_launch(s->ctx, 0);
     delete s;
     return 0;
 }

@@ -94,6 +94,11 @@ module pdmd_f
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: waves_per_system
     end function
+    integer(c_int) function pdmd_set_graph_launch(ctx, on) bind(C, name='pdmd_set_graph_launch')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
     integer(c_int) function pdmd_advance_to_output(ctx, nsys, ids, max_nevents) bind(C, name='pdmd_advance_to_output')
       import :: c_int, c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: ctx

@@ -85,3 +85,28 @@ def test_memory_budget_limits_admission():
     finally:
         srv.close()
         e.close()
+
+
+def test_graph_replays_equal_eager():
+    # the same chunked advances through replays of one captured launch (with
+    # host reads and copies between them) and through direct launches
+    systems = [_sys([12] * 6, 81), _sys([8] * 16, 82), _sys([16] * 10, 83, tstar=0.4)]
+    chunks = [700, 700, 700, 700, 1300]
+    out = []
+    for graph in (False, True):
+        e = pdmd.Engine(waves=4)
+        try:
+            if graph:
+                e.set_graph_launch(True)
+            ids = [e.add_system(s, seed=300 + k) for k, s in enumerate(systems)]
+            for sid in ids:
+                e.log_enable(sid, sum(chunks))
+            for n in chunks:
+                e.advance(ids, [n] * len(ids))
+                assert all(e.stats(sid).status == 0 for sid in ids)
+            out.append([(e.download(sid)["sv"].copy(), e.log_read(sid, sum(chunks))) for sid in ids])
+        finally:
+            e.close()
+    for (sv0, l0), (sv1, l1) in zip(*out):
+        assert np.array_equal(sv0.view(np.int64), sv1.view(np.int64))
+        assert np.array_equal(l0, l1)
