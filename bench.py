@@ -55,6 +55,11 @@ def bytes_per_event(k_up, k_dn):
     return 2.0 * (k_up + k_dn) * 53.0 + 6 * 4 + 2 * 48
 
 
+# FP64 operations of one pair prediction (free flight, minimum image, dot
+# products, discriminant, root; a sqrt and a division count as one each)
+FLOPS_PER_PREDICTION = 37.0
+
+
 def _cpu_worker(args):
     sysd, seed, nev, param_dir = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -233,6 +238,8 @@ def main():
     ev_per_launch = args.per_step * len(ids)
     avg_launch_s = float(np.mean(launch_ms)) / 1e3
     achieved = ev_per_launch * bpe / avg_launch_s / 1e9
+    # the root solves: 2 (K_up + K_dn) predictions per committed entry
+    fp64_gflops = ev_per_launch * 2.0 * (k_up + k_dn) * FLOPS_PER_PREDICTION / avg_launch_s / 1e9
     bpe_meas = measured_bytes_per_entry()
 
     # plain: the same systems one at a time
@@ -301,6 +308,14 @@ def main():
             "k_up": round(k_up, 2),
             "k_dn": round(k_dn, 2),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+        },
+        "fp64_root_solves": {
+            "achieved": round(fp64_gflops, 3),
+            "peak": 78600.0,
+            "unit": "GFLOP/s",
+            "frac": fp64_gflops / 78600.0,
+            "flops_per_prediction": FLOPS_PER_PREDICTION,
+            "predictions_per_entry": round(2.0 * (k_up + k_dn), 2),
         },
         "cpu_baseline": cpu,
         "setup_s": round(gen_s + setup_s, 2),

@@ -401,9 +401,12 @@ contains
       desc%seed = seed
       desc%canon = int(canon, c_int32_t)
       desc%list_capacity = 0
+      desc%flags = merge(1_c_int32_t, 0_c_int32_t, no_hbs /= 0) + merge(2_c_int32_t, 0_c_int32_t, no_h /= 0)
       desc%sv = c_loc(sv)
       desc%bptnr = c_null_ptr
-      m = min(N, 4 * clen(1) + 8)
+      ! the first two chains: same-chain and inter-chain pairs
+      m = N
+      if (nch >= 2) m = min(N, 4 * (clen(1) + clen(2)))
       n = m * (m - 1) / 2
       allocate(a(n), b(n), cd(n))
       n = 0

@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST = os.path.join(ROOT, "parallel-dmd-for-biomolecules_amd", "host")
 DRIVER = os.path.join(HOST, "pdmd_driver")
 NML = "&pdmd_run\n mode='{mode}', nchains1=3, chnln1=8, nchains2=2, chnln2=6, density=0.01, gen_seed=5,\n" \
-      " canon=1, seed=3, start='{start}', boxl={boxl}, waves={waves}, debugging=1 /\n"
+      " canon=1, seed=3, start='{start}', boxl={boxl}, waves={waves}, debugging=1{extra} /\n"
 
 
 @pytest.fixture(scope="module")
@@ -33,9 +33,9 @@ def driver():
     return DRIVER
 
 
-def _run(driver, work, mode, tstar=0.20, ncoll=1000, waves=1, start="helix", boxl=0.0):
+def _run(driver, work, mode, tstar=0.20, ncoll=1000, waves=1, start="helix", boxl=0.0, extra=""):
     with open(os.path.join(work, "pdmd.nml"), "w") as f:
-        f.write(NML.format(mode=mode, waves=waves, start=start, boxl=repr(float(boxl))))
+        f.write(NML.format(mode=mode, waves=waves, start=start, boxl=repr(float(boxl)), extra=extra))
     r = subprocess.run([driver], input=f"{tstar}\n{ncoll}\n", cwd=work, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
@@ -85,17 +85,24 @@ def test_generate_writes_reference_inputs(driver, tmp_path):
     assert hp1 == [0] * 24 + [1] * 8
 
 
-def test_codes_through_fortran_desc(driver, tmp_path):
+@pytest.mark.parametrize("flags", [{}, {"no_hbs": True}, {"no_h": True}])
+def test_codes_through_fortran_desc(driver, tmp_path, flags):
+    # the build flags as namelist switches reach the descriptor's flags
     work = _workdir(tmp_path)
     _run(driver, work, "generate")
-    out = _run(driver, work, "codes")
+    out = _run(driver, work, "codes", extra="".join(f", {k}=1" for k in flags))
     assert "bytes on the device" in out
     rows = np.loadtxt(os.path.join(work, "codes.txt"), dtype=np.int64)
     ref = _python_system()
     N, B, sv = _read_helix(work)
     sysd = dict(ref, sv=sv, boxl=B)
-    want = pdmd.pair_codes(sysd, rows[:, 0], rows[:, 1], species=[0, 0, 0, 1, 1])
+    want = pdmd.pair_codes(sysd, rows[:, 0], rows[:, 1], species=[0, 0, 0, 1, 1], **flags)
     assert np.array_equal(rows[:, 2], want)
+    # pairs of both chains are listed, so each flag shows: no 15 with
+    # -Dno_hbs, no 16 between the chains with -Dno_h
+    inter = (rows[:, 0] <= 32) != (rows[:, 1] <= 32)
+    assert (15 in rows[:, 2]) != bool(flags.get("no_hbs"))
+    assert (16 in rows[inter, 2]) != bool(flags.get("no_h"))
 
 
 def _records(path):
