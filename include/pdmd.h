@@ -131,6 +131,16 @@ int pdmd_advance_to_output(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, 
  * whose master/worker speculation (main.F90:484-1258) this replaces.
  * Default 1. */
 int pdmd_set_waves(pdmd_ctx* ctx, int32_t waves_per_system);
+/* Neighbour lists and every calendar entry rebuilt now (nbor.f:23-137 +
+ * events.f:11-127, the reference's nbor() and events() calls at
+ * main.F90:1170-1177): the listed systems move their time origin to the
+ * current time as the sync pseudo-event does (main.F90:1126-1187: positions
+ * advanced, times shifted, calendar rescaled) and take its update branch
+ * without the displacement test; the pending sync entry keeps its time.
+ * pdmd_system_add does the same at start-up (main.F90:389-424).  A forced
+ * update changes the later event sequence as an extra update would in the
+ * reference. */
+int pdmd_rebuild(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids);
 /* Launch mode of pdmd_advance*: with on != 0 the event-loop launch is
  * captured in a HIP graph once per launch shape (systems count, largest
  * system, budget, flags, wavefronts) and replayed while the shape stays the

@@ -58,6 +58,7 @@ def lib():
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_run.restype = ctypes.c_int64
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64]
+        L.orc_rebuild.argtypes = [ctypes.c_void_p]
         L.orc_nbeads.argtypes = [ctypes.c_void_p]
         L.orc_get_state.argtypes = [ctypes.c_void_p, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
         L.orc_stats_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcStats)]
@@ -118,6 +119,10 @@ class Oracle:
         if r < 0:
             raise RuntimeError(f"oracle self-check failed ({r})")
         return log[: min(log_cap, nev)] if log_cap else None
+
+    def rebuild(self):
+        """List update forced at the current time (orc_rebuild)."""
+        lib().orc_rebuild(self.h)
 
     def state(self):
         N = self.N

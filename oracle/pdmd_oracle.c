@@ -1555,6 +1555,28 @@ static void sync_event(orc_sys* S) {
     add_tbin(S, i);
 }
 
+/* A list update forced at the current time: the sync pseudo-event's shift of
+ * the time origin and its update branch (main.F90:1126-1187: positions to
+ * the current time, times shifted, calendar rescaled, nbor.f + events.f),
+ * without the displacement test, the rate adaptation or a new sync entry. */
+void orc_rebuild(orc_sys* S) {
+    int N = S->N;
+    double tf = S->tfalse;
+    S->t = S->t + tf;
+    for (int k = 1; k <= N + 3; ++k) S->tim[k] = S->tim[k] - tf;
+    for (int k = 1; k <= N; ++k) {
+        SV(S, 1, k) = SV(S, 1, k) + SV(S, 4, k) * tf;
+        SV(S, 2, k) = SV(S, 2, k) + SV(S, 5, k) * tf;
+        SV(S, 3, k) = SV(S, 3, k) + SV(S, 6, k) * tf;
+    }
+    S->tfalse = 0.0;
+    S->interval_max = S->interval * S->n_forced;
+    S->sortsize = S->interval_max / (double)NUMBIN;
+    S->tbin_off = 0.0;
+    S->nupdates++;
+    rebuild_all(S);
+}
+
 static void energy(const orc_sys* S, orc_obs* o);
 /* output pseudo-event, main.F90:1191-1246 */
 static void output_event(orc_sys* S) {

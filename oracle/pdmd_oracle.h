@@ -72,6 +72,10 @@ int32_t orc_nbeads(const orc_sys* s);
 /* Process exactly nev events.  Logs up to log_cap of them.  Returns the
  * number processed, or a negative status on a self-check failure. */
 int64_t orc_run(orc_sys* s, int64_t nev, orc_event* log, int64_t log_cap);
+/* A list update forced at the current time (the sync pseudo-event's origin
+ * shift and update branch, main.F90:1126-1187, without its displacement
+ * test, rate adaptation or new sync entry). */
+void orc_rebuild(orc_sys* s);
 
 void orc_get_state(const orc_sys* s, double* sv, int32_t* identity, int32_t* bptnr,
                    int32_t* extra_repuls /* N*4 */, double* tim /* N+3 */,

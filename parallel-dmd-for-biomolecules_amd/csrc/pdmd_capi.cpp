@@ -647,6 +647,32 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
 
 extern "C" {
 
+int pdmd_rebuild(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids) {
+    if (!ctx || nsys <= 0 || !ids) return fail("pdmd_rebuild: bad arguments");
+    if (nsys > 4096) return fail("pdmd_rebuild: at most 4096 systems per call");
+    std::vector<int> dids(nsys);
+    int max_n = 0, max_cap = 0;
+    for (int k = 0; k < nsys; ++k) {
+        HostSys* H = get_sys(ctx, ids[k]);
+        if (!H) return -1;
+        for (int q = 0; q < k; ++q)
+            if (ids[q] == ids[k]) return fail("pdmd_rebuild: duplicate system id");
+        dids[k] = ids[k] - 1;
+        max_n = std::max(max_n, H->N);
+        max_cap = std::max(max_cap, H->cap);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_ids.p, dids.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(engine_rebuild(ctx->d_sys.p, ctx->d_ids.p, nsys, max_n, engine_maxw(max_cap), ctx->lds_limit, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nsys; ++k) {
+        SysState chk;
+        HIPCHK(hipMemcpy(&chk, get_sys(ctx, ids[k])->st.p, sizeof chk, hipMemcpyDeviceToHost));
+        if (chk.status) return fail("pdmd_rebuild: system " + std::to_string(ids[k]) + " status " +
+                                    std::to_string(chk.status), -3);
+    }
+    return 0;
+}
+
 int pdmd_set_graph_launch(pdmd_ctx* ctx, int32_t on) {
     if (!ctx) return fail("pdmd_set_graph_launch: null context");
     ctx->graphs = on != 0;

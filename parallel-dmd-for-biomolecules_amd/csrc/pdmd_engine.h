@@ -19,6 +19,10 @@ hipError_t engine_prepare(size_t lds_limit);
 // to 4 wavefronts each, as many as fit in lds_limit).
 hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, size_t lds_limit,
                         hipStream_t s);
+// A list update forced at the current time for n systems (origin shift,
+// nbor + events), one workgroup of up to 4 wavefronts each.
+hipError_t engine_rebuild(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, size_t lds_limit,
+                          hipStream_t s);
 // Event loop: every listed system runs until its SysState::target.
 // budget_ticks: wall_clock64 ticks after which each system stops between
 // rounds (0 = run to the event targets); waves: wavefronts per system (1 to

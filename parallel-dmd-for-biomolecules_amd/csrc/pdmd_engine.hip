@@ -2999,6 +2999,57 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_setup(const SysDev* __rest
     if (threadIdx.x == 0) store_sc(S.st, c);
 }
 
+// A list update forced at the current time (pdmd_rebuild): the sync
+// pseudo-event's shift of the time origin and its update branch
+// (main.F90:1126-1187), then the lists and events() on every wavefront of
+// the system's workgroup.  No displacement test, no rate adaptation and no
+// new sync entry: the pending one keeps its shifted time.
+__global__ __launch_bounds__(64 * kSetupWaves) void k_rebuild(const SysDev* __restrict__ sys, const int* ids,
+                                                              int maxw) {
+    const int nw = (int)(blockDim.x >> 6), wave = wave_id();
+    if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
+    __syncthreads();
+    const SysDev& S = sdev();
+    if (lane_id() == 0)
+        *reinterpret_cast<Lds*>(smem + kLdOff + wave * kLdStride) = make_lds(S, maxw, 0, nw, wave);
+    __syncthreads();
+    const Lds& L = ldsv();
+    copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
+    __syncthreads();
+    const DevTables& T = tabs();
+    Sc& c = *L.msc();  // the master scalars (each wavefront's own copy is not loaded here)
+    if (threadIdx.x == 0) load_sc(S.st, c);
+    __syncthreads();
+    const int N = S.N;
+    const double tf = c.tfalse;
+    __syncthreads();  // every wavefront holds tf before the first lane moves the origin
+    for (int e = (int)threadIdx.x; e < N + 3; e += (int)blockDim.x) {
+        Leaf lf = gld(S.leaf + (e));
+        lf.tim = lf.tim - tf;
+        gst(S.leaf + (e), lf);
+    }
+    for (int k = (int)threadIdx.x; k < N; k += (int)blockDim.x) {
+        Rec b = gld(S.rec + (k));
+        b.x = b.x + b.vx * tf;
+        b.y = b.y + b.vy * tf;
+        b.z = b.z + b.vz * tf;
+        gst(S.rec + (k), b);
+    }
+    if (threadIdx.x == 0) {
+        c.t = c.t + tf;
+        c.tfalse = 0.0;
+        c.interval_max = c.interval * c.n_forced;
+        c.sortsize = c.interval_max / (double)kNumbin;
+        c.tbin_off = 0.0;
+        c.nupdates++;
+    }
+    __threadfence_block();
+    __syncthreads();
+    rebuild(S, T, L, c);
+    __syncthreads();
+    if (threadIdx.x == 0) store_sc(S.st, c);
+}
+
 // Pop up to kmax calendar entries in key order (wavefront 0): pair events
 // only after the first, since a pseudo-event runs alone.  Each popped entry
 // but the last is taken out of the min tree until the round's rescans.
@@ -3573,6 +3624,14 @@ hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n,
     hipLaunchKernelGGL(k_setup, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
     return hipGetLastError();
 }
+hipError_t engine_rebuild(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxw, size_t lds_limit,
+                          hipStream_t s) {
+    int nw = kSetupWaves;
+    while (nw > 1 && engine_lds_bytes(nblk_for(max_n), maxw, 0, nw) > lds_limit) --nw;
+    const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0, nw);
+    hipLaunchKernelGGL(k_rebuild, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
+    return hipGetLastError();
+}
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
                       size_t lds_limit, hipStream_t s, long long budget_ticks, int flags, int waves) {
     const int nblk = nblk_for(max_n);
@@ -3622,7 +3681,9 @@ hipError_t engine_observables(const SysDev* d_sys, int id, const int* cstart, co
 hipError_t engine_prepare(size_t lds_limit) {
     hipError_t e = hipFuncSetAttribute((const void*)k_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)k_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+    e = hipFuncSetAttribute((const void*)k_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
 }
 
 }  // namespace pdmd

@@ -94,6 +94,12 @@ module pdmd_f
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: waves_per_system
     end function
+    integer(c_int) function pdmd_rebuild(ctx, nsys, ids) bind(C, name='pdmd_rebuild')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsys
+      integer(c_int32_t), intent(in) :: ids(*)
+    end function
     integer(c_int) function pdmd_set_graph_launch(ctx, on) bind(C, name='pdmd_set_graph_launch')
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -131,7 +131,7 @@ EXPORTS = [
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
     "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
-    "pdmd_set_waves", "pdmd_autotune_waves", "pdmd_check", "pdmd_set_graph_launch",
+    "pdmd_set_waves", "pdmd_autotune_waves", "pdmd_check", "pdmd_set_graph_launch", "pdmd_rebuild",
 ]
 
 _lib = None
@@ -159,6 +159,7 @@ def lib():
         L.pdmd_last_launch_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
         L.pdmd_set_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.pdmd_set_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.pdmd_rebuild.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p]
         L.pdmd_check.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, ctypes.POINTER(Check)]
         L.pdmd_autotune_waves.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, ctypes.c_int64, _i32p, _f64p]
         L.pdmd_download.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p]
@@ -249,6 +250,12 @@ class Engine:
     def set_waves(self, waves):
         """Wavefronts per system for later advances (pdmd_set_waves)."""
         _check(lib().pdmd_set_waves(self._ctx, int(waves)))
+
+    def rebuild(self, ids):
+        """Neighbour lists and calendar rebuilt now, at each system's
+        current time (pdmd_rebuild)."""
+        a = np.ascontiguousarray(ids, np.int32)
+        _check(lib().pdmd_rebuild(self._ctx, len(a), _p(a, ctypes.c_int32)))
 
     def set_graph_launch(self, on=True):
         """Replay a captured HIP graph of the event-loop launch while the

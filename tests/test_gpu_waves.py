@@ -1,7 +1,8 @@
 """Several wavefronts per system: rounds of side-by-side pair events must
 commit exactly the serial event sequence.  Each system's event log and final
 state are compared bit for bit with the CPU oracle, for 2, 4 and 8
-wavefronts, on single systems and on a batch of uneven systems in one launch.
+wavefronts, on single systems and on a batch of uneven systems in one launch,
+and after a list update forced between events (pdmd_rebuild).
 """
 import numpy as np
 import pytest
@@ -12,11 +13,12 @@ from oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 
-def _compare(eng, sid, sysd, seed, n, param_dir, canon=True):
+def _compare(eng, sid, sysd, seed, n, param_dir, canon=True, orc=None):
     g = eng.log_read(sid, n)
     st = eng.stats(sid)
     assert st.status == 0
-    orc = Oracle(sysd, seed=seed, param_dir=param_dir, canon=canon)
+    if orc is None:
+        orc = Oracle(sysd, seed=seed, param_dir=param_dir, canon=canon)
     o = orc.run(n, log_cap=n)
     assert len(g) == n
     bad = np.nonzero((g["i"] != o["i"]) | (g["j"] != o["j"]) | (g["type"] != o["type"]) |
@@ -61,5 +63,28 @@ def test_uneven_batch_matches_serial(param_dir, waves):
         eng.advance(ids, [n] * len(ids))
         for k, (sid, s) in enumerate(zip(ids, systems)):
             _compare(eng, sid, s, 200 + k, n, param_dir)
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("waves", [1, 4])
+def test_forced_rebuild_matches_oracle(param_dir, waves):
+    # pdmd_rebuild between events (mid-interval, after an uneven number of
+    # entries) against the oracle's orc_rebuild at the same point
+    sysd = pdmd.generate([16] * 20, density=0.01, tstar=0.25, seed=33)
+    n1, n2 = 4321, 8000
+    eng = pdmd.Engine(waves=waves)
+    try:
+        sid = eng.add_system(sysd, seed=19)
+        eng.advance([sid], [n1])
+        up0 = eng.stats(sid).nupdates
+        eng.rebuild([sid])
+        assert eng.stats(sid).nupdates == up0 + 1 and eng.stats(sid).tfalse == 0.0
+        eng.log_enable(sid, n2)
+        eng.advance([sid], [n2])
+        orc = Oracle(sysd, seed=19, param_dir=param_dir)
+        orc.run(n1)
+        orc.rebuild()
+        _compare(eng, sid, sysd, 19, n2, param_dir, orc=orc)
     finally:
         eng.close()

@@ -172,3 +172,28 @@ def test_ghost_touches_one_bead(param_dir):
         assert list(changed) == [log["j"][0] - 1]
         return
     pytest.skip("no ghost within 200 events")
+
+
+def test_forced_rebuild_keeps_invariants(param_dir):
+    # a list update forced between events (orc_rebuild): the time origin
+    # moves to the current time, the lists and the calendar are rebuilt; energy
+    # stays conserved, every entry is again the brute-force minimum (now
+    # predicted at the new origin, so exactly), and the run goes on
+    sysd = _system([16] * 8, 51)
+    orc = Oracle(sysd, seed=6, param_dir=param_dir, canon=False)
+    e0 = orc.observables().ered
+    orc.run(7777)
+    s0 = orc.stats()
+    assert s0.tfalse > 0.0
+    orc.rebuild()
+    s1, st = orc.stats(), orc.state()
+    assert s1.tfalse == 0.0 and s1.t == s0.t + s0.tfalse and s1.nupdates == s0.nupdates + 1
+    for i in range(1, orc.N + 1):
+        best, who = _brute_tim(orc, i, st)
+        if best is None or best >= s1.interval_max:
+            assert st["tim"][i - 1] >= s1.interval_max or st["nptnr"][i - 1] == who
+            continue
+        assert st["tim"][i - 1] == best and st["nptnr"][i - 1] == who
+    assert abs(orc.observables().ered - e0) <= 1e-10 * abs(e0)
+    orc.run(30000)
+    assert abs(orc.observables().ered - e0) <= 1e-10 * abs(e0)
