@@ -18,9 +18,10 @@
 
 namespace pdmd {
 
-// topo word: chain (13 bits) | kind (2: 0 Calpha, 1 N, 2 C, 3 R) | residue-1
-// (7) | chain length-1 (7) | hp (1) | proline residue (1) | system built
-// with -Dno_hbs (1, the sign bit)
+// topo word: chain (12 bits: at most 4095 chains of >= 16 beads fit in
+// 65533) | system built with -Dno_h (1) | kind (2: 0 Calpha, 1 N, 2 C, 3 R)
+// | residue-1 (7) | chain length-1 (7) | hp (1) | proline residue (1) |
+// system built with -Dno_hbs (1, the sign bit)
 // dyn word:  identity (5 bits) | (H-bond partner + 1) << 5
 struct __attribute__((aligned(16))) Rec {
     double x, y, z, vx, vy, vz, bm;

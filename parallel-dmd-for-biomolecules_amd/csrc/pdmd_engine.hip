@@ -1350,7 +1350,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const bool batch = bk >= 0;
     int status = 0, nb = 0;
     PH_BEGIN(tph);
-    // ---- round 1: list rows of i and j, whole rows with their lengths
+    // ---- first loads: list rows of i and j, whole rows with their lengths
     const int nui = rw.nui, ndi = rw.ndi, nuj = rw.nuj, ndj = rw.ndj;
     const int eui = rw.eui, edi = rw.edi, euj = rw.euj, edj = rw.edj;
     const int xi0 = P.xi.x, xi1 = P.xi.y, xi2 = P.xi.z;
@@ -1464,7 +1464,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     if (spec) v_spec[myslot].l = l;
     wave_sync();
     PH_END(S, kPhScan, tph);
-    // ---- round 2: the down beads' own data, the rows of the batched beads
+    // ---- second loads: the down beads' own data, the rows of the batched beads
     //      and the candidates of i and j, all in flight together
     Rec rl = P.ri;
     int4 ol = make_int4(-1, -1, -1, -1), xl = make_int4(-1, -1, -1, -1);
