@@ -610,6 +610,7 @@ struct Batch {
     int ty[kMaxWaves];    // resolved type of each event
     int nbl[kMaxWaves];   // blocks to rescan, in each wavefront's list
     int st[kMaxWaves];    // status after each event
+    int wdone[kMaxWaves]; // the wavefront's committed entries are written
     int K;                // entries popped this round (0: stop)
     int cut;              // first wavefront whose event is not committed
     int rebuild;          // the round's sync event rebuilds the lists
@@ -747,6 +748,19 @@ __device__ inline Lds lds_regs() {
     return r;
 }
 __device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : gld(L.gleaf + k); }
+// a leaf another wavefront of the workgroup has just written (its HBM copy
+// read past the vector L1)
+__device__ inline Leaf leaf_get_fresh(const Lds& L, int k) {
+    if (L.leaf_lds) return L.lleaf()[k];
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(L.gleaf + k);
+    const unsigned long long a = __hip_atomic_load((GLB unsigned long long*)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long b = __hip_atomic_load((GLB unsigned long long*)(q + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Leaf r;
+    r.tim = __longlong_as_double((long long)a);
+    r.nptnr = (int)(unsigned)b;
+    r.meta = (int)(unsigned)(b >> 32);
+    return r;
+}
 __device__ inline void leaf_set(const Lds& L, int k, const Leaf& v) {
     if (L.leaf_lds)
         L.lleaf()[k] = v;
@@ -1545,20 +1559,33 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     }
     int sp = -1;  // the wavefront's bead of the shoulder set, marked in a round
     bool sp_w = false;
+    int rep = -1;  // earlier event whose calendar-entry write this bead's updates replay on
+    // the down bead's updates may take a full recompute (a pending op)
+    const bool mfull = first && !is_i && ((lf0.nptnr == i && cI > 0) || (lf0.nptnr == j && cJ > 0) || cI >= 2 || cJ >= 2);
+    // single-pair predictions of the down beads (placed after the pairs)
+    const bool wantI = first && cI > 0, wantJ = first && cJ > 0;
+    const unsigned long long mI = __ballot(wantI), mJ = __ballot(wantJ);
+    const int nI = __popcll(mI), nsing = nI + __popcll(mJ);
+    const bool sing_fit = npairs + nsing <= maxw;
     // ---- a round's footprint check.  Marks per bead: bit k = wavefront k
     //      rewrites the bead's state, bit 8+k = it rewrites the bead's
-    //      calendar entry.  Event k commits only if no earlier event of the
-    //      round rewrites a state it reads or an entry it rewrites; a later
-    //      event reads the state before this one's writes (they wait for the
-    //      round's last barrier), as in the serial order.
+    //      calendar entry, bit 16+k = that rewrite may be a full recompute.
+    //      Event k commits only if no earlier event of the round rewrites a
+    //      state it reads or an entry it rewrites; a later event reads the
+    //      state before this one's writes (they wait for the round's last
+    //      barrier), as in the serial order.  One exception: a down bead
+    //      whose entry exactly one earlier event rewrote by single-pair
+    //      updates only, and whose own updates here are single-pair ones,
+    //      replays those updates on that event's entry once it is written.
     if (batch) {
         PH_RESET(tph);
         Batch& B = *L.batch();
-        const unsigned sbit = 1u << bk, lbit = 1u << (8 + bk);
+        const unsigned sbit = 1u << bk, lbit = 1u << (8 + bk), fbit = 1u << (16 + bk);
         sp = shoulder_bead(S, P, lane);
-        if (dent >= 0) mark_or(S.mark + dent, lbit);
-        if (lane == 0) mark_or(S.mark + i, sbit | lbit);
-        if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit);
+        if (lane == 0) B.wdone[bk] = 0;
+        if (dent >= 0) mark_or(S.mark + dent, lbit | (mfull ? fbit : 0u));
+        if (lane == 0) mark_or(S.mark + i, sbit | lbit | fbit);
+        if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit | fbit);
         sp_w = (P.flags & kPairWroteSp) != 0;
         const bool sp_r = (P.flags & kPairReadsSp) != 0;
         if (bk == 0 && sp_w && sp >= 0) mark_or(S.mark + sp, sbit);
@@ -1574,15 +1601,26 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                     if (mark_get(S.mark + m) & lowS) bad = true;
                 }
             }
-            if (dent >= 0 && (mark_get(S.mark + dent) & (lowS | lowL))) bad = true;
+            if (dent >= 0) {
+                const unsigned mk = mark_get(S.mark + dent);
+                if (mk & lowS) {
+                    bad = true;
+                } else if ((mk & lowL) && first) {
+                    // (other lanes of the same bead leave it to its first)
+                    const unsigned wl = (mk & lowL) >> 8;
+                    const int m = __ffs(wl) - 1;
+                    if (!is_i && __popc(wl) == 1 && !(mk & (1u << (16 + m))) && !mfull && sing_fit)
+                        rep = m;
+                    else
+                        bad = true;
+                }
+            }
             if (lane == 0 && (mark_get(S.mark + i) & (lowS | lowL))) bad = true;
             if (lane == 1 && j >= 0 && (mark_get(S.mark + j) & (lowS | lowL))) bad = true;
             if (sp_r && sp >= 0 && (mark_get(S.mark + sp) & lowS)) bad = true;
             // a down bead whose full recompute is not in the batch would
             // read rows that were not checked
-            const bool need_full = first && !is_i &&
-                                   ((lf0.nptnr == i && cI > 0) || (lf0.nptnr == j && cJ > 0) || cI >= 2 || cJ >= 2);
-            if (need_full && !(spec && v_spec[myslot].off >= 0)) bad = true;
+            if (mfull && !(spec && v_spec[myslot].off >= 0)) bad = true;
             if (__ballot(bad) && lane == 0) atomicMin(&B.cut, bk);
         }
         PH_END(S, kPhBCheck, tph);
@@ -1597,10 +1635,6 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     // ---- the single-pair predictions (l, i) and (l, j) of each down bead,
     //      after the pair entries: each is the same for every op of l with
     //      that bead, so one per (bead, partner) serves the whole op sequence
-    const bool wantI = first && cI > 0, wantJ = first && cJ > 0;
-    const unsigned long long mI = __ballot(wantI), mJ = __ballot(wantJ);
-    const int nI = __popcll(mI), nsing = nI + __popcll(mJ);
-    const bool sing_fit = npairs + nsing <= maxw;
     int sqI = -1, sqJ = -1;
     if (sing_fit) {
         const unsigned long long below = (1ull << lane) - 1ull;
@@ -1854,15 +1888,57 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             gst(S.rec + (j), P.rj);
         }
     }
-    if (wr) leaf_set(L, l, lf);
+    bool wb = wr;  // the lane's bead takes a new entry (its block is rescanned)
+    if (rep >= 0) {
+        // the earlier event's entry first, then this event's single-pair
+        // updates on it (i before j, eventredo_down.f), as the serial order
+        // would have applied them
+        Batch& B = *L.batch();
+        while (__hip_atomic_load(&B.wdone[rep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+            __builtin_amdgcn_s_sleep(1);
+        const Leaf cur = leaf_get_fresh(L, l);
+        double rt = cur.tim;
+        int rp = cur.nptnr, rc = leaf_ct(cur.meta), st2 = 0;
+        bool mod = false;
+        if (cI > 0) {
+            const double t = v_wt[sqI] + tf;
+            if (t < rt) {
+                rt = t;
+                rp = i;
+                rc = v_wy[sqI];
+                mod = true;
+            }
+        }
+        if (cJ > 0) {
+            const double t = v_wt[sqJ] + tf;
+            if (t < rt) {
+                rt = t;
+                rp = j;
+                rc = v_wy[sqJ];
+                mod = true;
+            }
+        }
+        if (mod) {
+            Leaf nl;
+            nl.tim = rt;
+            nl.nptnr = rp;
+            nl.meta = make_meta(calc_bin(c, rt, false, st2), rc);
+            leaf_set(L, l, nl);
+        }
+        st2 = wave_status(st2);
+        if (st2 && lane == 0 && B.st[bk] == 0) B.st[bk] = st2;
+        wb = mod;
+    } else if (wr) {
+        leaf_set(L, l, lf);
+    }
     if (lane == 0) {
         L.blist()[0] = i / kBlk;
         L.blist()[1] = j >= 0 ? j / kBlk : i / kBlk;
         L.blist()[2] = extra_blk >= 0 ? extra_blk : i / kBlk;
     }
-    const unsigned long long mm = __ballot(wr);
+    const unsigned long long mm = __ballot(wb);
     const int pre = __popcll(mm & ((1ull << lane) - 1ull));
-    if (wr) L.blist()[3 + pre] = l / kBlk;
+    if (wb) L.blist()[3 + pre] = l / kBlk;
     const int nbl = 3 + __popcll(mm);
     PH_END(S, kPhDown, tph);
     if (lane == 0) {
@@ -1890,8 +1966,14 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         gs1(&S.extra[ncim1].z, -1);
     }
     if (batch) {
-        // the round's block rescans run after every committed event's writes
-        if (lane == 0) L.batch()->nbl[bk] = nbl;
+        // the round's block rescans run after every committed event's
+        // writes; a later event that replays on these entries waits for
+        // the flag
+        wave_sync_mem();
+        if (lane == 0) {
+            L.batch()->nbl[bk] = nbl;
+            __hip_atomic_store(&L.batch()->wdone[bk], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
         return;
     }
     if (!L.leaf_lds || xpulse_del) {
