@@ -2690,6 +2690,152 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     wg_sync(nw);
 }
 
+// Neighbour lists of a small system (N <= kDenseN) without the cell sort.
+// Positions, topology words, override slots and fine cells are staged in
+// LDS (wavefront 0's work area is free during a rebuild); each wavefront
+// takes beads in turn, tests the fine-cell stencil of all candidates one per
+// lane, and queues the (bead, candidate) pairs that pass; full 64-lane
+// batches of the queue then take the pair-code and cutoff test.  Membership
+// is rebuild_lists' own (the n_wrap = 2 stencil, then the bonded/local codes
+// or the cutoff, nbor.f:63-128); rows come in bead-index order.
+// Wavefront 0's area holds at least 24 * maxw >= 18432 bytes; the staging
+// takes 60 bytes per bead and its queue 512 bytes after it.
+constexpr int kDenseN = 256;
+constexpr int kDenseQ = 128;  // queue ring: under 64 entries + one pass of 64
+struct __attribute__((aligned(16))) Stage {
+    double x, y, z;
+    int topo, pad;
+};
+__device__ void rebuild_lists_dense(const SysDev&, const Lds&, Sc& c) {
+    const Lds& L = ldsv();
+
+    const SysDev& S = sdev();
+
+    const DevTables& T = tabs();
+    const int lane = lane_id(), wave = wave_id(), nw = L.nw;
+    const int tid = (int)threadIdx.x, nthr = kW * nw;
+    const int N = S.N, M = S.M, cap = S.cap;
+    PH_BEGIN(trl);
+    unsigned char* const w0 = smem + reinterpret_cast<const Lds*>(smem + kLdOff)->o_wm;
+    Stage* const stg = reinterpret_cast<Stage*>(w0);
+    int4* const sovr = reinterpret_cast<int4*>(w0 + 32 * N);
+    int2* const cnt = reinterpret_cast<int2*>(w0 + 48 * N);
+    int* const sfine = reinterpret_cast<int*>(w0 + 56 * N);
+    int* const q = wave == 0 ? reinterpret_cast<int*>(w0 + 60 * N) : reinterpret_cast<int*>(smem + L.o_wm);
+    for (int k = tid; k < N; k += nthr) {
+        const Rec b = gld(S.rec + (k));
+        int cx = (int)((b.x + S.half) / S.width);
+        int cy = (int)((b.y + S.half) / S.width);
+        int cz = (int)((b.z + S.half) / S.width);
+        cx = ((cx % M) + M) % M;
+        cy = ((cy % M) + M) % M;
+        cz = ((cz % M) + M) % M;
+        Stage g;
+        g.x = b.x;
+        g.y = b.y;
+        g.z = b.z;
+        g.topo = b.topo;
+        g.pad = 0;
+        stg[k] = g;
+        sovr[k] = gld(S.ovr + (k));
+        sfine[k] = cx | (cy << 10) | (cz << 20);
+    }
+    for (int i = wave + nw * lane; i < N; i += nw * kW) cnt[i] = make_int2(0, 0);
+    wg_sync(nw);
+    PH_END(S, kPhLCells, trl);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int qh = 0, qn = 0;  // queue head and length (wavefront-uniform)
+    // the pair test on the next min(qn, 64) queued pairs, in queue order
+    auto drain = [&]() {
+        const int nb = min(qn, kW);
+        const bool v = lane < nb;
+        const int e = v ? q[(qh + lane) & (kDenseQ - 1)] : 0;
+        const int i = e >> 16, jb = e & 0xFFFF;
+        bool take = false;
+        int stat = 0;
+        if (v) {
+            const Stage si = stg[i], sj = stg[jb];
+            const int4 oi = sovr[i];
+            stat = static_t(sj.topo, si.topo);
+            // ev_code(j,i), nbor.f:63: j's overrides exist only towards i's
+            int code = stat;
+            if ((oi.x >= 0 && (oi.x >> 8) == jb) || (oi.y >= 0 && (oi.y >> 8) == jb) ||
+                (oi.z >= 0 && (oi.z >> 8) == jb) || (oi.w >= 0 && (oi.w >> 8) == jb))
+                code = dyn_code(sovr[jb], i, stat);
+            if ((code >= 4 && code <= 12) || (code >= 17 && code < 27)) {
+                take = true;
+            } else {
+                double rx = si.x - sj.x, ry = si.y - sj.y, rz = si.z - sj.z;
+                rx = rx - round(rx);
+                ry = ry - round(ry);
+                rz = rz - round(rz);
+                const double r2 = rx * rx + ry * ry + rz * rz;
+                take = code >= 1 && code <= 50 && r2 <= T.rlsq[code];
+            }
+        }
+        const bool tu = take && jb > i, td = take && jb < i;
+        const unsigned long long up = __ballot(tu), dn = __ballot(td);
+        // a bead's pairs are contiguous in the queue: lanes [s, lane) of
+        // this lane's segment precede it in its row
+        const int ip = __shfl_up(i, 1);
+        const unsigned long long hm = __ballot(v && (lane == 0 || ip != i));
+        const unsigned long long hl = hm & ((2ull << lane) - 1ull);
+        const int sh = hl ? 63 - __clzll(hl) : 0;
+        const unsigned long long seg = below & ~((1ull << sh) - 1ull);
+        const bool tail = v && (lane + 1 == nb || ((hm >> (lane + 1)) & 1ull));
+        int2 b0 = make_int2(0, 0);
+        if (v) b0 = cnt[i];
+        if (tu) {
+            const int p = b0.x + __popcll(up & seg);
+            if (p < cap) gs1(S.nb_up + ((size_t)i * cap + p), jb | (stat << 24));
+        }
+        if (td) {
+            const int p = b0.y + __popcll(dn & seg);
+            if (p < cap) gs1(S.nb_dn + ((size_t)i * cap + p), jb | (stat << 24));
+        }
+        const unsigned long long segi = seg | (1ull << lane);
+        if (tail) cnt[i] = make_int2(b0.x + __popcll(up & segi), b0.y + __popcll(dn & segi));
+        qh = (qh + nb) & (kDenseQ - 1);
+        qn -= nb;
+    };
+    for (int i = wave; i < N; i += nw) {
+        const int fi = sfine[i];
+        const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
+        for (int base = 0; base < N; base += kW) {
+            const int jb = base + lane;
+            bool pass = false;
+            if (jb < N && jb != i) {
+                const int fj = sfine[jb];
+                // fine cells lie in [0, M): differences in (-M, M), taken mod M
+                int dx = (fj & 1023) - ix, dy = ((fj >> 10) & 1023) - iy, dz = (fj >> 20) - iz;
+                dx += dx < 0 ? M : 0;
+                dy += dy < 0 ? M : 0;
+                dz += dz < 0 ? M : 0;
+                pass = !((dx > 2 && dx < M - 2) || (dy > 2 && dy < M - 2) || (dz > 2 && dz < M - 2));
+            }
+            const unsigned long long m = __ballot(pass);
+            if (pass) q[(qh + qn + __popcll(m & below)) & (kDenseQ - 1)] = (i << 16) | jb;
+            qn += __popcll(m);
+            if (qn >= kW) drain();
+        }
+    }
+    while (qn > 0) drain();
+    int mx = 0, status = 0;
+    for (int i = wave + nw * lane; i < N; i += nw * kW) {
+        const int2 n2 = cnt[i];
+        gs1(S.nup + (i), min(n2.x, cap));
+        gs1(S.ndn + (i), min(n2.y, cap));
+        if (n2.x > cap || n2.y > cap) status = kErrListCapacity;
+        mx = max(mx, max(n2.x, n2.y));
+    }
+    PH_END(S, kPhLSearch, trl);
+    status = wave_status(status);
+    if (status && !c.status) c.status = status;
+    mx = wave_imax(mx);
+    if (lane == 0) atomicMax(&S.st->max_nb_seen, (long long)mx);
+    wg_sync(nw);
+}
+
 // events.f:11-127 with tfalse = 0: every bead's next event, then the
 // calendar; all wavefronts of the system take part (the master scalars'
 // scan position is at bin 1 already)
@@ -2782,7 +2928,10 @@ __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
     }
     wg_sync(L.nw);
     PH_BEGIN(trb);
-    rebuild_lists(S, L, c);
+    if (S.N <= kDenseN)
+        rebuild_lists_dense(S, L, c);
+    else
+        rebuild_lists(S, L, c);
     PH_END(S, kPhLists, trb);
     events_all(S, T, L, c);
     PH_END(S, kPhEvents, trb);
