@@ -1,6 +1,8 @@
 """Device error paths, forced from the host: a list capacity too small for
 the system's neighbour lists (the rebuild's kErrListCapacity status comes back
-as an error of pdmd_system_add), and calls on ids that do not exist.  The
+as an error of pdmd_system_add), for a 512-bead system (lists built through
+the cell sort) and a 128-bead one (lists built by the all-pairs stencil
+filter of systems up to 256 beads), and calls on ids that do not exist.  The
 context stays usable after each error, at 1 and 4 wavefronts."""
 import pytest
 
@@ -9,9 +11,10 @@ import pdmd
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("chains", [[16] * 8, [8] * 4], ids=["512beads", "128beads"])
 @pytest.mark.parametrize("waves", [1, 4])
-def test_list_capacity_overflow_is_reported(waves):
-    sysd = pdmd.generate([16] * 8, density=0.01, tstar=0.20, seed=11)
+def test_list_capacity_overflow_is_reported(waves, chains):
+    sysd = pdmd.generate(chains, density=0.01, tstar=0.20, seed=11)
     eng = pdmd.Engine(waves=waves)
     try:
         with pytest.raises(RuntimeError, match="status 1"):

@@ -1,0 +1,51 @@
+"""Neighbour lists on the device against the oracle's (nbor.f:23-137
+restated): right after set-up, the longest up or down row and the total row
+lengths equal the oracle's; after 20000 entries (several list rebuilds at
+the syncs) the totals still do.  Systems of up to 256 beads build their lists
+with the all-pairs stencil filter, larger ones through the cell sort; both
+are covered, at 1 and 4 wavefronts."""
+import numpy as np
+import pytest
+
+import pdmd
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_rows(orc, n):
+    # the oracle numbers beads from 1, as the reference does
+    up = np.array([orc.nb_up(i).size for i in range(1, n + 1)])
+    dn = np.array([orc.nb_dn(i).size for i in range(1, n + 1)])
+    return up, dn
+
+
+@pytest.mark.parametrize("chains,density,tstar", [
+    ([8] * 4, 0.0101, 0.27),     # 128 beads
+    ([12] * 5, 0.02, 0.43),      # 240 beads
+    ([4] * 16, 0.03, 0.35),      # 256 beads, the largest of the stencil filter
+    ([8] * 9, 0.012, 0.30),      # 288 beads, cell sort
+    ([16] * 8, 0.01, 0.20),      # 512 beads, cell sort
+], ids=["128", "240", "256", "288", "512"])
+@pytest.mark.parametrize("waves", [1, 4])
+def test_list_lengths_match_oracle(param_dir, chains, density, tstar, waves):
+    sysd = pdmd.generate(chains, density=density, tstar=tstar, seed=77)
+    eng = pdmd.Engine(waves=waves)
+    try:
+        sid = eng.add_system(sysd, seed=5)
+        n = eng.nbeads[sid]
+        orc = Oracle(sysd, seed=5, param_dir=param_dir)
+        up, dn = _oracle_rows(orc, n)
+        st = eng.stats(sid)
+        assert st.status == 0
+        assert st.max_list == max(up.max(), dn.max())
+        assert round(st.avg_up * n) == up.sum() and round(st.avg_dn * n) == dn.sum()
+        eng.advance([sid], [20000])
+        orc.run(20000)
+        st = eng.stats(sid)
+        assert st.status == 0 and st.nupdates > 0
+        assert st.nupdates == orc.stats().nupdates
+        up, dn = _oracle_rows(orc, n)
+        assert round(st.avg_up * n) == up.sum() and round(st.avg_dn * n) == dn.sum()
+    finally:
+        eng.close()
