@@ -2698,10 +2698,14 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
 // batches of the queue then take the pair-code and cutoff test.  Membership
 // is rebuild_lists' own (the n_wrap = 2 stencil, then the bonded/local codes
 // or the cutoff, nbor.f:63-128); rows come in bead-index order.
-// Wavefront 0's area holds at least 24 * maxw >= 18432 bytes; the staging
-// takes 60 bytes per bead and its queue 512 bytes after it.
+// Wavefront 0's area holds at least 24 * maxw bytes (its work lists, maxw >=
+// kMinMaxw); the staging takes 60 bytes per bead and its queue 512 bytes
+// after it.
+constexpr int kMinMaxw = 768;
 constexpr int kDenseN = 256;
 constexpr int kDenseQ = 128;  // queue ring: under 64 entries + one pass of 64
+static_assert(60 * kDenseN + 4 * kDenseQ <= 24 * kMinMaxw, "dense list staging exceeds wavefront 0's work area");
+static_assert(4 * kDenseQ <= 4 * kMinMaxw, "dense list queue exceeds a wavefront's work list");
 struct __attribute__((aligned(16))) Stage {
     double x, y, z;
     int topo, pad;
@@ -3828,7 +3832,7 @@ __global__ void k_predict(const SysDev* sys, int id, int n, const int* pi, const
 
 // also sized for the list rebuild's per-lane cell ranges (27 x 64 int2 over
 // the work-list arrays)
-int engine_maxw(int cap) { return max((4 + kMaxSpec) * (cap + 3), 768); }
+int engine_maxw(int cap) { return max((4 + kMaxSpec) * (cap + 3), kMinMaxw); }
 
 static size_t r16(size_t b) { return (b + 15) / 16 * 16; }
 // the bytes make_lds lays out before the leaves, for nw wavefronts
