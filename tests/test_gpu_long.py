@@ -1,7 +1,7 @@
 """Long parity windows: twelve seeded random systems, six of up to 256 beads
 and six of up to 1536
 (thermostat on, so ghost events interleave) advance 3·10^5 calendar entries
-each in one batched launch at 4 wavefronts per system, and every entry of
+each in one batched launch at 2, 4 and 8 wavefronts per system, and every entry of
 each event log, times compared bitwise, and the final state equal the CPU
 oracle's.  The window spans hundreds of list rebuilds per small system (the
 all-pairs stencil filter up to 256 beads, the cell sort above) and many
@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 N_ENTRIES = 300000
 
 
-def test_long_window_batch(param_dir):
+@pytest.mark.parametrize("waves", [2, 4, 8])
+def test_long_window_batch(param_dir, waves):
     rng = np.random.default_rng(20261202)
     specs = []
     for k in range(12):
@@ -26,7 +27,7 @@ def test_long_window_batch(param_dir):
         specs.append(dict(chains=[L] * nch, density=float(rng.uniform(0.006, 0.02)),
                           tstar=float(rng.uniform(0.15, 0.5)), gen_seed=700 + k, run_seed=900 + k))
     systems = [pdmd.generate(s["chains"], density=s["density"], tstar=s["tstar"], seed=s["gen_seed"]) for s in specs]
-    eng = pdmd.Engine(waves=4)
+    eng = pdmd.Engine(waves=waves)
     try:
         ids = [eng.add_system(sysd, seed=s["run_seed"]) for s, sysd in zip(specs, systems)]
         for sid in ids:
