@@ -96,8 +96,16 @@ int pdmd_ctx_destroy(pdmd_ctx* ctx);
 int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* desc, int32_t* sys_id);
 int pdmd_system_remove(pdmd_ctx* ctx, int32_t sys_id);
 int pdmd_nbeads(pdmd_ctx* ctx, int32_t sys_id);
-/* Device bytes a system of this description would occupy. */
+/* Device bytes a system of this description would occupy: every buffer
+ * pdmd_system_add allocates for it (lists at the description's capacity,
+ * the coarse cell grid of its box and cutoffs), each rounded up to 4 KiB;
+ * the event log of pdmd_log_enable comes on top.  Host-only.  This form
+ * reads the parameter tables that ship beside the library (parameters/);
+ * pdmd_system_bytes_ctx uses the context's tables.  The serving loop's
+ * memory-budget admission uses the latter.  No counterpart in the reference,
+ * whose arrays are sized at compile time (code/header.f:11-40). */
 int64_t pdmd_system_bytes(const pdmd_system_desc* desc);
+int64_t pdmd_system_bytes_ctx(pdmd_ctx* ctx, const pdmd_system_desc* desc);
 
 /* Commit nevents[k] more calendar entries of each listed system, all
  * systems concurrently in one launch (one workgroup per system, of the

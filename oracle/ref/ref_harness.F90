@@ -4,9 +4,12 @@
 ! golden vectors for the CPU oracle.  Nothing here is shipped or measured.
 !
 ! stdin: line 1 T* (as temp_XXX line 1, main.F90:124-127), line 2 the number
-! of pair events of the serial loop below.  Reads the reference's input files
-! from the working directory (parameters/, parametersep/, helix.inp start via
-! -Drunh, inputinfo.f:28-66) and writes raw little-endian stream files:
+! of calendar entries of the serial loop below, line 3 (optional, default 1)
+! the seed of the project RNG that stands in for IFPORT drandm in the
+! terminal-bead capture draw (main.F90:1508-1515).  Reads the reference's
+! input files from the working directory (parameters/, parametersep/,
+! helix.inp start via -Drunh, inputinfo.f:28-66) and writes raw
+! little-endian stream files:
 !   setup.bin  scaled tables, bond tables, masses, cell geometry, sv
 !   evcode.bin ev_code(N,N) as int32, column-major
 !   nb.bin     na_npt(N), nnabdn(N), then each bead's up list and down list
@@ -14,14 +17,24 @@
 !   pred.bin   for every pair i<j (row-major in i): tij, type as events.f
 !              dispatches them (events.f:26-47)
 !   loop.bin   per committed entry (i, j, resolved coltype, tfalse)
-!   loopn.bin  entries, pair events, neighbour-list rebuilds, forced rebuilds
-!   final.bin  sv, tim, nptnr, coltype after the loop
+!   loopn.bin  entries, pair events, neighbour-list rebuilds, forced
+!              rebuilds, capture draws, then the tally of resolved coltypes
+!              1..30 (main.F90:926)
+!   final.bin  sv, tim, nptnr, coltype, tfalse, identity, bptnr,
+!              extra_repuls after the loop
 !   obs.bin    energy/radgyr/end_to_end before and after the loop
+!   restore.bin identity, bptnr, extra_repuls, ev_code after the -Drunr
+!              restart restore (main.F90:240-321) of the final state
 !
-! The serial loop commits calendar entries in the reference's order: bead
-! pairs whose outcome needs no resolution by the master/worker code of
-! main.F90 (coltype 1-3, and 4/8 between two side chains), and the sync and
-! output pseudo-events.  It stops at the first pair event outside that set.
+! The serial loop commits calendar entries in the reference's order: the
+! earliest entry of the first non-empty bin, first in list order on ties
+! (main.F90:496-545 with one worker), every bead pair event through the
+! worker's resolution (main.F90:1432-1960, restated below around calls of
+! the reference's repuls_check, repuls_check_3, check_sigma, eventdyn,
+! bumpoff, repuls_add, repuls_del_a) and the master's partial_events call
+! (main.F90:943), and the sync and output pseudo-events.  It runs without
+! the Andersen thermostat (tim(N+1) = 1e9), as the reference does without
+! -Dcanon.
 
 #include "def.h"
 #include "header.f"
@@ -36,11 +49,21 @@
 	integer, allocatable :: coltypes(:)
 	real*8 tstar_in, tij, tmin, ered, tred, sumvel, ehh_ii, ehh_ij, rg_avg, e2e_avg
 	integer hb_alpha, hb_ii, hb_ij
-	logical direct, over
+	logical over
 	integer boundbad, unboundbad
 	real*8, allocatable :: svk(:,:)
 	real*8 ev_rec(2)
 	integer ev_int(3)
+	integer(8) rng_seed, rng_ctr1
+	integer ios, tally(30)
+	integer, allocatable :: ident0(:), bptnr_fin(:)
+	real*8 rxr, ryr, rzr, diffr
+	interface
+		real(8) function ref_u01(seed, stream, ctr) bind(C, name='ref_u01')
+		integer(8), value :: seed, ctr
+		integer(4), value :: stream
+		end function
+	end interface
 
 	noptotal = nop1 + nop2
 	coll = 0
@@ -50,6 +73,11 @@
 	fname_digits_pre = '0000'
 	read(5,*) tstar_in
 	read(5,*) nsteps
+	rng_seed = 1
+	read(5,*,iostat=ios) rng_seed
+	if (ios /= 0) rng_seed = 1
+	rng_ctr1 = 0
+	tally = 0
 	setemp = tstar_in*12.d0
 	pi = 4.d0*datan(1.d0)
 	! main.F90:144-149
@@ -61,6 +89,8 @@
 
 	call inputinfo()
 	call make_code()
+	allocate(ident0(noptotal), bptnr_fin(noptotal))
+	ident0(1:noptotal) = identity(1:noptotal)
 
 	! main.F90:196-230
 	t = 0.d0
@@ -185,9 +215,9 @@
 
 	! serial loop: selection as main.F90:496-545 with one worker (earliest
 	! entry of the first non-empty bin, first in list order on ties), bead
-	! pair events committed directly (eventdyn + partial_events, main.F90:
-	! 1636,926,943), the sync pseudo-event as main.F90:1126-1187 and the
-	! output pseudo-event's reschedule as main.F90:1238-1242
+	! pair events through resolve_pair below, the sync pseudo-event as
+	! main.F90:1126-1187 and the output pseudo-event's reschedule as
+	! main.F90:1238-1242
 	open(unit=21, file='loop.bin', access='stream', form='unformatted', status='replace')
 	npair = 0
 	nlog = 0
@@ -202,23 +232,14 @@
 			k = tlinks(k)
 		end do
 		j = 0
-		if (i <= noptotal) then
-			j = nptnr(i)
-			typ = coltype(i)
-			direct = (typ >= 1 .and. typ <= 3)
-			if ((typ == 4 .or. typ == 8) .and. identity(i) >= 9 .and. identity(j) >= 9) direct = .true.
-			if (.not. direct) exit
-		else if (i == noptotal+1) then
-			exit
-		endif
+		if (i == noptotal+1) exit
 		tim_sel = tim(i)
 		tfalse = tim(i)
 		coll = coll + 1
 		if (i <= noptotal) then
-			evcode = ev_code(i,j)
-			call eventdyn(i,j,evcode)
-			typ = coltype(i)
-			call partial_events(i,j,.false.)
+			j = nptnr(i)
+			call resolve_pair(i, j, typ)
+			if (typ >= 1 .and. typ <= 30) tally(typ) = tally(typ) + 1
 			npair = npair + 1
 		else if (i == noptotal+2) then
 			typ = 0
@@ -280,12 +301,15 @@
 	end do
 	close(21)
 	open(unit=21, file='loopn.bin', access='stream', form='unformatted', status='replace')
-	write(21) nlog, npair, nupdates_h, nforced_h
+	write(21) nlog, npair, nupdates_h, nforced_h, int(rng_ctr1)
+	write(21) tally
 	close(21)
 
 	open(unit=21, file='final.bin', access='stream', form='unformatted', status='replace')
 	write(21) sv, tim, nptnr, coltype, tfalse
+	write(21) identity, bptnr, extra_repuls
 	close(21)
+	bptnr_fin(1:noptotal) = bptnr(1:noptotal)
 
 	call energy(ered,tred,sumvel,hb_alpha,hb_ii,hb_ij,ehh_ii,ehh_ij)
 	call radgyr(rg_avg)
@@ -317,7 +341,257 @@
 	call checkover(over)
 	write(6,'(a,l2)') 'CHECKEND ', over
 	sv(:, 1:noptotal) = svk(:, 1:noptotal)
+
+	! -Drunr restart from the final state: the positions as a restart reads
+	! them back (wrapped by inputinfo.f:90-95 and again by main.F90:210-216),
+	! identities and codes from the input files (inputinfo.f, make_code.f),
+	! bptnr from the .bptnr record, then the restore of main.F90:240-321:
+	! every pair k < kj in order, an N-C pair inside its well gets its
+	! shoulders (repuls_add) unless an end bead takes part, and the pair
+	! (k, bptnr(k)) takes the bound identities 5/8.  check_nc_int.f then
+	! checks the restored state on fresh lists.
+	do k = 1, noptotal
+		do kk = 1, 3
+			sv(kk,k) = sv(kk,k) - dnint(sv(kk,k))
+			sv(kk,k) = sv(kk,k) - dnint(sv(kk,k))
+		end do
+		identity(k) = ident0(k)
+		bptnr(k) = bptnr_fin(k)
+		do kk = 1, 4
+			extra_repuls(k,kk) = 0
+		end do
+	end do
+	call make_code()
+	do k = 1, noptotal-1
+		do kk = k+1, noptotal
+			if (identity(k)+identity(kk) .eq. 5 .and. ev_code(k,kk) .eq. 15) then
+				rxr = sv(1,k) - sv(1,kk)
+				ryr = sv(2,k) - sv(2,kk)
+				rzr = sv(3,k) - sv(3,kk)
+				rxr = rxr - dnint(rxr)
+				ryr = ryr - dnint(ryr)
+				rzr = rzr - dnint(rzr)
+				diffr = rxr*rxr + ryr*ryr + rzr*rzr - welldia_sq(identity(k),identity(kk))
+				if (diffr .lt. 0.d0 .and. .not. is_end(k) .and. .not. is_end(kk)) then
+					if (identity(k) .eq. 1) then
+						call repuls_add(k,kk)
+					else
+						call repuls_add(kk,k)
+					endif
+				endif
+			endif
+			if (kk .eq. bptnr(k)) then
+				if (identity(k) .eq. 1) then
+					identity(k) = 5
+					identity(kk) = 8
+				else
+					identity(k) = 8
+					identity(kk) = 5
+				endif
+			endif
+		end do
+	end do
+	tfalse = 0.d0
+	do k = 1, noptotal
+		old_rx(k) = sv(1,k)
+		old_ry(k) = sv(2,k)
+		old_rz(k) = sv(3,k)
+	end do
+	call nbor()
+	write(6,'(a)') 'CHECK restored'
+	boundbad = 0
+	unboundbad = 0
+	call check_nc_int(boundbad, unboundbad)
+	write(6,'(a,l2,2i8)') 'CHECKEND ', .false., boundbad, unboundbad
+	do j = 1, noptotal
+		do i = 1, noptotal
+			code32((j-1)*noptotal + i) = ev_code(i,j)
+		end do
+	end do
+	open(unit=21, file='restore.bin', access='stream', form='unformatted', status='replace')
+	write(21) identity, bptnr, extra_repuls, code32
+	close(21)
+
 	write(6,'(a,i8,a,f10.4)') 'harness: pair events committed ', npair, ' avg nbors ', dble(nbrsum)/dble(noptotal)
+
+	contains
+
+	! end bead of its chain as main.F90:1483-1494 and 1493-1494 test it:
+	! the N bead of the first residue or the C bead of the last
+	logical function is_end(k)
+	integer, intent(in) :: k
+	integer kl
+	if (k .le. nop1) then
+		kl = k - (chnnum(k)-1)*numbeads1
+		is_end = (kl .eq. chnln1+1) .or. (kl .eq. 3*chnln1)
+	else
+		kl = k - nop1 - (chnnum(k)-(nop1/numbeads1)-1)*numbeads2 + numbeads1
+		is_end = (kl .eq. numbeads1+chnln2+1) .or. (kl .eq. numbeads1+3*chnln2)
+	endif
+	end function is_end
+
+	! One bead pair event as a worker resolves it (main.F90:1432-1960) on
+	! the current state, then the master's commit (main.F90:641-943, whose
+	! table writes repeat the worker's in one address space) and
+	! partial_events.  The capture draw of an end-bead N-C pair uses the
+	! project RNG (stream 1) in place of drandm.
+	subroutine resolve_pair(i, j, typ)
+	integer, intent(in) :: i, j
+	integer, intent(out) :: typ
+	integer evc, hbp, x
+	logical ends, xdel
+	real*8 rating, u
+	evc = ev_code(i,j)
+	ends = is_end(i) .or. is_end(j)
+	xdel = .false.
+	if (coltype(i) .eq. 7) then
+		if ((extra_repuls(i,4) .eq. 0) .and. (extra_repuls(j,4) .eq. 0)) then
+			if (.not. ends) then
+				if (identity(i) < identity(j)) then
+					call repuls_check(i,j,rating)
+				else
+					call repuls_check(j,i,rating)
+				endif
+				if (rating .le. 10.d0) then
+					coltype(i) = 4
+				else
+					coltype(i) = 14
+				endif
+			else
+				u = ref_u01(rng_seed, 1, rng_ctr1)
+				rng_ctr1 = rng_ctr1 + 1
+				if (u .le. 0.2d0) then
+					coltype(i) = 4
+				else
+					coltype(i) = 9
+				endif
+			endif
+		else
+			coltype(i) = 9
+		endif
+	elseif (coltype(i) .eq. 10) then
+		if (evc < 45) then
+			hbp = extra_repuls(i,4)
+			if (identity(i) < identity(hbp)) then
+				call repuls_check_3(i,hbp,j,rating)
+			else
+				call repuls_check_3(hbp,i,j,rating)
+			endif
+		else
+			hbp = extra_repuls(j,4)
+			if (identity(j) < identity(hbp)) then
+				call repuls_check_3(j,hbp,i,rating)
+			else
+				call repuls_check_3(hbp,j,i,rating)
+			endif
+		endif
+		if (rating .le. 10.d0) then
+			coltype(i) = 5
+		else
+			coltype(i) = 15
+		endif
+	elseif (coltype(i) .eq. 12) then
+		if (evc < 45) then
+			x = i
+		else
+			x = j
+		endif
+		hbp = extra_repuls(x,4)
+		if (bptnr(x) == hbp) then
+			call check_sigma(x,hbp,rating)
+			if (rating .gt. 10.d0) then
+				coltype(i) = 6
+			else
+				coltype(i) = 13
+			endif
+		else
+			coltype(i) = 15
+		endif
+	endif
+	if (coltype(i) .lt. 14) call eventdyn(i,j,evc)
+	if (coltype(i) .eq. 20) then
+		if (identity(i)+identity(j) .eq. 5) then
+			bptnr(i) = j
+			bptnr(j) = i
+			identity(i) = identity(i)+4
+			identity(j) = identity(j)+4
+			if (.not. ends) then
+				if (identity(i) .lt. identity(j)) then
+					call repuls_add(i,j)
+				else
+					call repuls_add(j,i)
+				endif
+			endif
+		endif
+	elseif (coltype(i) .eq. 21) then
+		if (identity(i) .le. 8) then
+			if (.not. ends) then
+				if (identity(i) .lt. identity(j)) then
+					call repuls_del_a(i,j)
+				else
+					call repuls_del_a(j,i)
+				endif
+				xdel = .true.
+			endif
+			if (bptnr(i) == j) then
+				bptnr(i) = 0
+				bptnr(j) = 0
+				identity(i) = identity(i)-4
+				identity(j) = identity(j)-4
+			endif
+		endif
+	elseif (coltype(i) .eq. 24) then
+		if (evc < 45) then
+			x = i
+		else
+			x = j
+		endif
+		hbp = extra_repuls(x,4)
+		if (identity(x) + identity(hbp) .eq. 5) then
+			bptnr(x) = hbp
+			bptnr(hbp) = x
+			identity(x) = identity(x)+4
+			identity(hbp) = identity(hbp)+4
+		endif
+	elseif (coltype(i) .eq. 25) then
+		if (evc < 45) then
+			x = i
+		else
+			x = j
+		endif
+		hbp = extra_repuls(x,4)
+		if (identity(x) .ge. 5) then
+			bptnr(x) = 0
+			bptnr(hbp) = 0
+			identity(x) = identity(x)-4
+			identity(hbp) = identity(hbp)-4
+		endif
+	elseif (coltype(i) .eq. 14) then
+		call bumpoff(i,j,evc)
+		if (.not. ends) then
+			if (identity(i) .lt. identity(j)) then
+				call repuls_add(i,j)
+			else
+				call repuls_add(j,i)
+			endif
+		endif
+	elseif (coltype(i) .eq. 15) then
+		call bumpoff(i,j,evc)
+	elseif (coltype(i) .eq. 16) then
+		call bumpoff(i,j,evc)
+		if (.not. ends) then
+			if (identity(i) .lt. identity(j)) then
+				call repuls_del_a(i,j)
+			else
+				call repuls_del_a(j,i)
+			endif
+			xdel = .true.
+		endif
+	endif
+	typ = coltype(i)
+	call partial_events(i,j,xdel)
+	end subroutine resolve_pair
+
 	end program ref_harness
 
 #include "add_tbin.f"
@@ -348,5 +622,7 @@
 #include "radgyr.f"
 #include "end2end.f"
 #include "repuls_check.f"
+#include "repuls_check_3.f"
+#include "check_sigma.f"
 #include "checkover.f"
 #include "check_nc_int.f"

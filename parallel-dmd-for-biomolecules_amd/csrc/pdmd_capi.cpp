@@ -1,6 +1,7 @@
 // pdmd_capi.cpp -- C ABI of the MI355X PRIME20 DMD engine (include/pdmd.h):
 // contexts, system upload with the reference's start-up sequence, device
 // launches, downloads, event logs and observables.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -178,87 +179,23 @@ int validate_desc(const pdmd_system_desc* d) {
     return 0;
 }
 
-}  // namespace
-}  // namespace pdmd
+// Host half of the start-up of pdmd_system_add: validation, topology, the
+// -Drunr H-bond restore, tables, cutoffs and cell geometry.  No device call.
+struct Prep {
+    std::unique_ptr<HostSys> hs;
+    std::vector<Rec> bead;
+    std::vector<int4> extra, ovr;
+    std::vector<double2> bw;
+    DevTables tab{};
+    SysState st{};
+    int M = 0, Mq = 0, nres = 0;
+    double setemp = 0;
+};
 
-using namespace pdmd;
-
-extern "C" {
-
-const char* pdmd_version(void) { return "pdmd-mi355x 0.1 (gfx950)"; }
-const char* pdmd_last_error(void) { return g_err.c_str(); }
-
-int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
-    if (!out || !param_dir) return fail("pdmd_ctx_create: null argument");
-    auto ctx = std::make_unique<pdmd_ctx>();
-    std::string err;
-    if (!load_params(ctx->raw, param_dir, err)) return fail(err);
-    ctx->param_dir = param_dir;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device", -3);
-    HIPCHK(hipGetDevice(&ctx->device));
-    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ctx->ev0));
-    HIPCHK(hipEventCreate(&ctx->ev1));
-    // LDS per workgroup: opt in to the device maximum (160 KB on gfx950)
-    {
-        int lds = 0, optin = 0;
-        (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
-        (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device);
-        size_t want = (size_t)std::max(lds, optin);
-        if (want < 65536) want = 65536;
-        if (engine_prepare(want) == hipSuccess)
-            ctx->lds_limit = want;
-        else {
-            (void)hipGetLastError();
-            ctx->lds_limit = (size_t)std::max(lds, 32768);
-        }
-    }
-    {
-        int khz = 0;
-        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
-            ctx->wall_khz = khz;
-        else
-            (void)hipGetLastError();
-    }
-    *out = ctx.release();
-    return 0;
-}
-
-int pdmd_ctx_destroy(pdmd_ctx* ctx) {
-    if (!ctx) return 0;
-    if (ctx->stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
-    }
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-    delete ctx;
-    return 0;
-}
-
-int64_t pdmd_system_bytes(const pdmd_system_desc* d) {
-    if (validate_desc(d)) return -1;
-    int64_t N = 0, nres = 0;
-    for (int c = 0; c < d->nchains; ++c) {
-        N += 4 * d->chain_len[c];
-        nres += d->chain_len[c];
-    }
-    const int64_t cap = d->list_capacity > 0 ? d->list_capacity : 64;
-    int64_t b = N * (int64_t)sizeof(Rec) + N * 16 /*bw*/ + 2 * N * 16 /*extra, ovr*/ + N * 32 /*old_r*/ +
-                2 * N * cap * 4 + 2 * N * 4 + (N + 3) * 20 + nres * 48 + (int64_t)sizeof(DevTables);
-    // coarse grid upper bound: box/rl cells per axis at the densest tables
-    b += 4 * N * 4 + N * 48 + 2 * 4 * 1024 * 1024;
-    b += N * 4;  // footprint marks
-    return b;
-}
-
-int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
-    if (!ctx || !sys_id) return fail("pdmd_system_add: null argument");
+int prepare_system(const Params& raw, const pdmd_system_desc* d, Prep& out) {
     if (int e = validate_desc(d)) return e;
-    auto hs = std::make_unique<HostSys>();
-    HostSys& H = *hs;
+    out.hs = std::make_unique<HostSys>();
+    HostSys& H = *out.hs;
     const int nc = d->nchains;
     std::vector<int> clen(d->chain_len, d->chain_len + nc), spec(nc);
     for (int c = 0; c < nc; ++c) spec[c] = d->species ? d->species[c] : c;
@@ -283,11 +220,12 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     H.boxl = d->boxl;
     H.tstar = d->tstar;
     H.seed = d->seed;
-    H.p = ctx->raw;
+    H.p = raw;
     scale_params(H.p, d->boxl);
     const Params& P = H.p;
     // state: positions wrapped as inputinfo.f:90-95 and main.F90:210-216 do
-    std::vector<Rec> bead(N);
+    std::vector<Rec>& bead = out.bead;
+    bead.assign(N, Rec{});
     for (int k = 0; k < N; ++k) {
         Rec b{};
         double x = d->sv[6 * k], y = d->sv[6 * k + 1], z = d->sv[6 * k + 2];
@@ -303,11 +241,14 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
         b.vx = d->sv[6 * k + 3];
         b.vy = d->sv[6 * k + 4];
         b.vz = d->sv[6 * k + 5];
-        b.bm = ctx->raw.bmass[t.ident0[k]];  // inputinfo.f:400-403, initial identities
+        b.bm = raw.bmass[t.ident0[k]];  // inputinfo.f:400-403, initial identities
         bead[k] = b;
     }
     std::vector<int> ident = t.ident0, bptnr(N, -1);
-    std::vector<int4> extra(N, make_int4(-1, -1, -1, -1)), ovr(N, make_int4(-1, -1, -1, -1));
+    std::vector<int4>& extra = out.extra;
+    std::vector<int4>& ovr = out.ovr;
+    extra.assign(N, make_int4(-1, -1, -1, -1));
+    ovr.assign(N, make_int4(-1, -1, -1, -1));
     if (d->bptnr)
         for (int k = 0; k < N; ++k) {
             const int b = d->bptnr[k];
@@ -401,7 +342,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     if (M >= 1024) return fail("pdmd_system_add: box too large for packed cell coordinates");
     const int Mq = M / 2;
     // device tables
-    DevTables tab{};
+    DevTables& tab = out.tab;
+    tab = DevTables{};
     for (int a = 0; a < 29; ++a)
         for (int b = 0; b < 29; ++b) {
             tab.sigma_sq[a][b] = P.sigma_sq[a][b];
@@ -416,7 +358,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     tab.epsilon1 = P.epsilon[1];
     // per-bead topology/identity words and own side-chain bond windows
     // (inputinfo.f:300-349, bond.f:21-31: Calpha owns code 10, N code 11, C code 12)
-    std::vector<double2> bw(N, make_double2(0.0, 0.0));
+    std::vector<double2>& bw = out.bw;
+    bw.assign(N, make_double2(0.0, 0.0));
     for (int k = 0; k < N; ++k) {
         const int c = t.chain[k], L = t.chain_len[c], kind = (t.loc[k] - 1) / L, r = (t.loc[k] - 1) % L + 1;
         const int q = t.res[k];
@@ -425,13 +368,14 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
         bead[k].dyn = make_dyn(ident[k], bptnr[k]);
         if (kind <= 2) {
             const int iii = rt[q] - 8;
-            const double bl = (kind == 0 ? ctx->raw.drca[iii] : kind == 1 ? ctx->raw.drnh[iii] : ctx->raw.drco[iii]) /
+            const double bl = (kind == 0 ? raw.drca[iii] : kind == 1 ? raw.drnh[iii] : raw.drco[iii]) /
                               P.boxl_orig;
-            const double dl = kind == 0 ? ctx->raw.del_rca[iii] : kind == 1 ? ctx->raw.del_rnh[iii] : ctx->raw.del_rco[iii];
+            const double dl = kind == 0 ? raw.del_rca[iii] : kind == 1 ? raw.del_rnh[iii] : raw.del_rco[iii];
             bw[k] = make_double2((1.0 - dl) * bl, (1.0 + dl) * bl);
         }
     }
-    SysState st{};
+    SysState& st = out.st;
+    st = SysState{};
     const double setemp = d->tstar * 12.0;  // main.F90:126-127
     st.t_fact = 0.00005;
     st.n_forced = 150.0;
@@ -439,6 +383,137 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     st.interval_max = st.n_forced * st.interval;
     st.sortsize = st.interval_max / (double)kNumbin;
     st.nbin = 1;
+    out.M = M;
+    out.Mq = Mq;
+    out.nres = nres;
+    out.setemp = setemp;
+    return 0;
+}
+
+// Device bytes of one prepared system: every buffer pdmd_system_add
+// allocates, each rounded up to the allocator's 4 KiB granularity.
+int64_t device_bytes(const Prep& pp) {
+    const HostSys& H = *pp.hs;
+    const int64_t N = H.N, cap = H.cap, nc = (int64_t)pp.Mq * pp.Mq * pp.Mq;
+    const int64_t sizes[] = {(int64_t)sizeof(DevTables), N * (int64_t)sizeof(Rec), N * 16 /*bw*/, N * 16 /*extra*/,
+                             N * 16 /*ovr*/, N * 32 /*old_r*/, N * cap * 4 /*nb_up*/, N * cap * 4 /*nb_dn*/,
+                             N * 4 /*nup*/, N * 4 /*ndn*/, (N + 3) * (int64_t)sizeof(Leaf), N * 4 /*cfine*/,
+                             N * 4 /*ckey*/, N * 4 /*csorted*/, N * 32 /*spos*/, N * 16 /*sinfo*/, N * 4 /*mark*/,
+                             (nc + 1) * 4 /*cstart*/, nc * 4 /*crun*/, (int64_t)sizeof(SysState)};
+    int64_t b = 0;
+    for (int64_t x : sizes) b += (x + 4095) / 4096 * 4096;
+    return b;
+}
+
+// the parameter tables that ship beside the library (parameters/)
+const Params* shipped_params() {
+    static Params p;
+    static int state = 0;  // 0 unread, 1 loaded, -1 failed
+    if (state == 0) {
+        state = -1;
+        Dl_info info{};
+        if (dladdr((const void*)&shipped_params, &info) && info.dli_fname) {
+            std::string dir(info.dli_fname);
+            const size_t cut = dir.find_last_of('/');
+            dir = (cut == std::string::npos ? std::string(".") : dir.substr(0, cut)) + "/parameters";
+            std::string err;
+            if (load_params(p, dir, err)) state = 1;
+        }
+    }
+    return state == 1 ? &p : nullptr;
+}
+
+}  // namespace
+}  // namespace pdmd
+
+using namespace pdmd;
+
+extern "C" {
+
+const char* pdmd_version(void) { return "pdmd-mi355x 0.1 (gfx950)"; }
+const char* pdmd_last_error(void) { return g_err.c_str(); }
+
+int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
+    if (!out || !param_dir) return fail("pdmd_ctx_create: null argument");
+    auto ctx = std::make_unique<pdmd_ctx>();
+    std::string err;
+    if (!load_params(ctx->raw, param_dir, err)) return fail(err);
+    ctx->param_dir = param_dir;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("no HIP device", -3);
+    HIPCHK(hipGetDevice(&ctx->device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ctx->ev0));
+    HIPCHK(hipEventCreate(&ctx->ev1));
+    // LDS per workgroup: opt in to the device maximum (160 KB on gfx950)
+    {
+        int lds = 0, optin = 0;
+        (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+        (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device);
+        size_t want = (size_t)std::max(lds, optin);
+        if (want < 65536) want = 65536;
+        if (engine_prepare(want) == hipSuccess)
+            ctx->lds_limit = want;
+        else {
+            (void)hipGetLastError();
+            ctx->lds_limit = (size_t)std::max(lds, 32768);
+        }
+    }
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
+            ctx->wall_khz = khz;
+        else
+            (void)hipGetLastError();
+    }
+    *out = ctx.release();
+    return 0;
+}
+
+int pdmd_ctx_destroy(pdmd_ctx* ctx) {
+    if (!ctx) return 0;
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    delete ctx;
+    return 0;
+}
+
+int64_t pdmd_system_bytes(const pdmd_system_desc* d) {
+    const Params* raw = shipped_params();
+    if (!raw) return fail("pdmd_system_bytes: parameter tables beside the library not found");
+    Prep pp;
+    if (int e = prepare_system(*raw, d, pp)) return e;
+    return device_bytes(pp);
+}
+
+int64_t pdmd_system_bytes_ctx(pdmd_ctx* ctx, const pdmd_system_desc* d) {
+    if (!ctx) return fail("pdmd_system_bytes_ctx: null context");
+    Prep pp;
+    if (int e = prepare_system(ctx->raw, d, pp)) return e;
+    return device_bytes(pp);
+}
+
+int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
+    if (!ctx || !sys_id) return fail("pdmd_system_add: null argument");
+    Prep pp;
+    if (int e = prepare_system(ctx->raw, d, pp)) return e;
+    // an advance may still be running on the context's stream: it reads the
+    // descriptor table and the id buffer that the start-up below rewrites
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HostSys& H = *pp.hs;
+    const int N = H.N, nres = pp.nres, M = pp.M, Mq = pp.Mq;
+    const double setemp = pp.setemp;
+    const std::vector<Rec>& bead = pp.bead;
+    const std::vector<int4>& extra = pp.extra;
+    const std::vector<int4>& ovr = pp.ovr;
+    const std::vector<double2>& bw = pp.bw;
+    const DevTables& tab = pp.tab;
+    const SysState& st = pp.st;
     {
         std::vector<DevTables> tv(1, tab);
         HIPCHK(upload(H.tab, tv));
@@ -503,27 +578,25 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.st = H.st.p;
     D.log = nullptr;
     H.alive = true;
-    // slot
+    // slot: the first free one; the system is registered there only once
+    // its start-up has succeeded
     int id = -1;
     for (size_t k = 0; k < ctx->sys.size(); ++k)
         if (!ctx->sys[k]) {
             id = (int)k;
             break;
         }
-    if (id < 0) {
-        id = (int)ctx->sys.size();
-        ctx->sys.emplace_back();
-    }
-    ctx->sys[id] = std::move(hs);
+    const size_t nslots = id < 0 ? ctx->sys.size() + 1 : ctx->sys.size();
+    if (id < 0) id = (int)ctx->sys.size();
     // device descriptor table
-    if (ctx->d_sys.n < ctx->sys.size()) {
+    if (ctx->d_sys.n < nslots) {
         DevBuf<SysDev> nb;
-        HIPCHK(nb.alloc(ctx->sys.size() + 16));
+        HIPCHK(nb.alloc(nslots + 16));
         if (ctx->d_sys.p) HIPCHK(hipMemcpy(nb.p, ctx->d_sys.p, sizeof(SysDev) * ctx->d_sys.n, hipMemcpyDeviceToDevice));
         std::swap(ctx->d_sys.p, nb.p);
         std::swap(ctx->d_sys.n, nb.n);
     }
-    HIPCHK(hipMemcpy(ctx->d_sys.p + id, &ctx->sys[id]->dev, sizeof(SysDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_sys.p + id, &H.dev, sizeof(SysDev), hipMemcpyHostToDevice));
     // device start-up: lists, pseudo-event seeds, events()
     if (ctx->d_ids.n < 1) HIPCHK(ctx->d_ids.alloc(4096));
     if (ctx->d_inc.n < 1) HIPCHK(ctx->d_inc.alloc(4096));
@@ -532,13 +605,14 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     SysState chk;
     HIPCHK(hipMemcpy(&chk, H.st.p, sizeof chk, hipMemcpyDeviceToHost));
-    if (chk.status) {
-        ctx->sys[id].reset();
+    if (chk.status)
         return fail("pdmd_system_add: device start-up failed with status " + std::to_string(chk.status), -4);
-    }
+    if ((size_t)id == ctx->sys.size()) ctx->sys.emplace_back();
+    ctx->sys[id] = std::move(pp.hs);
     *sys_id = id + 1;
     return 0;
 }
+
 
 static HostSys* get_sys(pdmd_ctx* ctx, int32_t id) {
     if (!ctx || id < 1 || id > (int)ctx->sys.size() || !ctx->sys[id - 1]) {

@@ -200,7 +200,7 @@ int pdmd_server_submit(pdmd_server* s, const pdmd_system_desc* d, int64_t ncoll,
     r->desc.hp = r->hp.data();
     r->desc.sv = r->sv.data();
     r->desc.bptnr = d->bptnr ? r->bptnr.data() : nullptr;
-    r->bytes = pdmd_system_bytes(&r->desc);
+    r->bytes = pdmd_system_bytes_ctx(s->ctx, &r->desc);
     if (r->bytes <= 0 || r->bytes > s->budget) return serve_fail("pdmd_server_submit: request exceeds the memory budget");
     r->ncoll = ncoll;
     r->id = s->next_id++;

@@ -82,6 +82,11 @@ module pdmd_f
       import :: c_int64_t, pdmd_system_desc
       type(pdmd_system_desc), intent(in) :: desc
     end function
+    integer(c_int64_t) function pdmd_system_bytes_ctx(ctx, desc) bind(C, name='pdmd_system_bytes_ctx')
+      import :: c_int64_t, c_ptr, pdmd_system_desc
+      type(c_ptr), value :: ctx
+      type(pdmd_system_desc), intent(in) :: desc
+    end function
     integer(c_int) function pdmd_advance(ctx, nsys, ids, nevents) bind(C, name='pdmd_advance')
       import :: c_int, c_ptr, c_int32_t, c_int64_t
       type(c_ptr), value :: ctx

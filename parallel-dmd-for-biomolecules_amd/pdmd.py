@@ -126,7 +126,7 @@ class Check(ctypes.Structure):
 
 EXPORTS = [
     "pdmd_version", "pdmd_last_error", "pdmd_ctx_create", "pdmd_ctx_destroy", "pdmd_system_add",
-    "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
+    "pdmd_system_remove", "pdmd_nbeads", "pdmd_system_bytes", "pdmd_system_bytes_ctx", "pdmd_advance", "pdmd_advance_async", "pdmd_wait",
     "pdmd_download", "pdmd_stats", "pdmd_log_enable", "pdmd_log_read", "pdmd_predict_pairs", "pdmd_observables",
     "pdmd_generate", "pdmd_generate_seq", "pdmd_last_launch_ms", "pdmd_pair_codes",
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
@@ -153,6 +153,8 @@ def lib():
         L.pdmd_system_remove.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.pdmd_nbeads.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.pdmd_system_bytes.argtypes = [ctypes.POINTER(SystemDesc)]
+        L.pdmd_system_bytes_ctx.restype = ctypes.c_int64
+        L.pdmd_system_bytes_ctx.argtypes = [ctypes.c_void_p, ctypes.POINTER(SystemDesc)]
         L.pdmd_advance.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
         L.pdmd_advance_async.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i64p]
         L.pdmd_wait.argtypes = [ctypes.c_void_p]

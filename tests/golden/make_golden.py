@@ -14,6 +14,7 @@ system, in a working directory of reference-format input files:
 and stores inputs and outputs as .npz (data only).  Run in this container
 after `make -C oracle/ref`; the GPU box never runs it.
 """
+import hashlib
 import os
 import re
 import shutil
@@ -32,6 +33,22 @@ OUT = os.path.join(ROOT, "oracle", "_ref")
 # (the sizes oracle/ref/Makefile compiles the checker for)
 L1, L2, NCH1, NCH2 = 8, 6, 2, 2
 LOG_KEEP = 20000
+# the whole serial loop is pinned event by event through digests of
+# consecutive blocks of this many entries (block_digests below)
+BLOCK = 1000
+LOG_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("t", "<f8")])
+
+
+def block_digests(i, j, typ, t, block=BLOCK):
+    """8-byte BLAKE2b digest of each block of `block` consecutive log
+    entries, packed as (i, j, type int32, t float64) little-endian records;
+    the tests digest the oracle's and the device's logs the same way."""
+    rec = np.zeros(len(i), LOG_DTYPE)
+    rec["i"], rec["j"], rec["type"], rec["t"] = i, j, typ, t
+    raw = rec.tobytes()
+    step = block * LOG_DTYPE.itemsize
+    return np.array([np.frombuffer(hashlib.blake2b(raw[k:k + step], digest_size=8).digest(), "<u8")[0]
+                     for k in range(0, len(raw), step)], np.uint64)
 
 
 def write_inputs(work, sysd, seq1, seq2):
@@ -121,20 +138,31 @@ def read_outputs(work, N, L1, L2):
     p = np.fromfile(os.path.join(work, "pred.bin"), np.dtype([("tij", "<f8"), ("type", "<i4")]))
     o["pred_tij"], o["pred_type"] = p["tij"].copy(), p["type"].copy()
     ln = np.fromfile(os.path.join(work, "loopn.bin"), np.int32)
-    o["loop_n"], o["loop_pairs"], o["loop_updates"], o["loop_forced"] = (int(x) for x in ln[:4])
-    lp = np.fromfile(os.path.join(work, "loop.bin"), np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"),
-                                                               ("t", "<f8")]))
-    # the first LOG_KEEP entries are kept event by event; the final state
-    # pins the whole run
+    o["loop_n"], o["loop_pairs"], o["loop_updates"], o["loop_forced"], o["loop_draws"] = (int(x) for x in ln[:5])
+    # resolved coltypes 1..30 over the whole loop (tally[k-1] = type k)
+    o["loop_tally"] = ln[5:35].copy()
+    lp = np.fromfile(os.path.join(work, "loop.bin"), LOG_DTYPE)
+    # the first LOG_KEEP entries are kept event by event, the whole loop as
+    # block digests; the final state pins it as well
     for k in ("i", "j", "type", "t"):
         o["loop_" + k] = lp[k][:LOG_KEEP].copy()
+    o["loop_digest"] = block_digests(lp["i"], lp["j"], lp["type"], lp["t"])
     fin = np.fromfile(os.path.join(work, "final.bin"), np.uint8)
     q = 0
     o["sv1"] = np.frombuffer(fin[q:q + 48 * N].tobytes(), np.float64); q += 48 * N
     o["tim1"] = np.frombuffer(fin[q:q + 8 * (N + 3)].tobytes(), np.float64); q += 8 * (N + 3)
     o["nptnr1"] = np.frombuffer(fin[q:q + 4 * (N + 3)].tobytes(), np.int32); q += 4 * (N + 3)
     o["coltype1"] = np.frombuffer(fin[q:q + 4 * (N + 3)].tobytes(), np.int32); q += 4 * (N + 3)
-    o["tfalse1"] = np.frombuffer(fin[q:q + 8].tobytes(), np.float64)[0]
+    o["tfalse1"] = np.frombuffer(fin[q:q + 8].tobytes(), np.float64)[0]; q += 8
+    o["identity1"] = np.frombuffer(fin[q:q + 4 * N].tobytes(), np.int32); q += 4 * N
+    o["bptnr1"] = np.frombuffer(fin[q:q + 4 * N].tobytes(), np.int32); q += 4 * N
+    o["extra1"] = np.frombuffer(fin[q:q + 16 * N].tobytes(), np.int32).reshape(N, 4, order="F"); q += 16 * N
+    assert q == fin.size, (q, fin.size)
+    # -Drunr restore of the final state (main.F90:240-321)
+    rs = np.fromfile(os.path.join(work, "restore.bin"), np.int32)
+    o["rst_identity"], o["rst_bptnr"] = rs[:N].copy(), rs[N:2 * N].copy()
+    o["rst_extra"] = rs[2 * N:6 * N].reshape(N, 4, order="F").copy()
+    o["rst_ev_code"] = rs[6 * N:].reshape(N, N, order="F").copy()
     ob = np.fromfile(os.path.join(work, "obs.bin"), np.uint8).tobytes()
     rec = np.dtype([("f", "<f8", 7), ("i", "<i4", 3)])
     obs = np.frombuffer(ob, rec)
@@ -158,8 +186,8 @@ def read_checks(stdout):
         elif line.startswith("CHECKEND") and sec:
             f = line.split()[1:]
             out["chk_" + sec] = np.array(rows, np.int32).reshape(-1, 3)
-            if sec == "final":
-                out["chk_final_flags"] = np.array([f[0] == "T", int(f[1]), int(f[2]), noss], np.int32)
+            if sec in ("final", "restored"):
+                out["chk_" + sec + "_flags"] = np.array([f[0] == "T", int(f[1]), int(f[2]), noss], np.int32)
             else:
                 out["chk_" + sec + "_over"] = np.int32(f[0] == "T")
             sec = None
@@ -173,7 +201,7 @@ def read_checks(stdout):
     return out
 
 
-def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000):
+def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1):
     import pdmd
     rng = np.random.default_rng(seed)
     seq1 = [int(x) for x in rng.integers(9, 29, L1)]
@@ -185,13 +213,14 @@ def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000):
         shutil.rmtree(work)
     os.makedirs(work)
     write_inputs(work, sysd, seq1, seq2)
-    r = subprocess.run([exe], input=f"{tstar}\n{nsteps}\n", cwd=work, capture_output=True, text=True)
+    r = subprocess.run([exe], input=f"{tstar}\n{nsteps}\n{rng_seed}\n", cwd=work, capture_output=True, text=True)
     if r.returncode != 0:
         raise SystemExit(f"{exe} failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
     print(name, r.stdout.strip().splitlines()[-1])
     N = 4 * (NCH1 * L1 + NCH2 * L2)
     o = read_outputs(work, N, L1, L2)
     o.update(read_checks(r.stdout))
+    o.update(rng_seed=np.uint64(rng_seed))
     o.update(chain_len=np.array([L1] * NCH1 + [L2] * NCH2, np.int32), restype=np.array(seq, np.int32),
              hp=sysd["hp"], boxl=sysd["boxl"], tstar=tstar, density=density, species=np.array([0] * NCH1 + [1] * NCH2,
                                                                                                np.int32))
@@ -209,6 +238,10 @@ def main():
     run_case("ref_2species", exe, seed=4242)
     run_case("ref_2species_nohbs", exe_n, seed=4243)
     run_case("ref_2species_noh", exe_h, seed=4244)
+    # dense and cool enough for N-C hydrogen bonds to form and break: every
+    # branch of the worker's resolution (coltypes 7, 10, 12 -> 4/5/6/9/13/
+    # 14/15/16 -> 20-27), shoulders added and removed, end-bead capture draws
+    run_case("ref_2species_hb", exe, seed=21, tstar=0.18, density=0.30, nsteps=1000000)
 
 
 if __name__ == "__main__":

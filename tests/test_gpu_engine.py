@@ -63,3 +63,31 @@ def test_event_sequence_first_10k(engine, param_dir, canon):
     assert np.array_equal(gs["sv"].view(np.int64), os_["sv"].view(np.int64))
     assert np.array_equal(gs["tim"].view(np.int64), os_["tim"].view(np.int64))
     engine.remove_system(sid)
+
+
+@pytest.mark.parametrize("waves", [1, 4])
+def test_add_system_while_an_advance_runs(param_dir, waves):
+    """pdmd_system_add waits for the advance in flight on the context's
+    stream before it rewrites the launch's id buffer and descriptor table: a
+    system added right after an asynchronous advance of another leaves both
+    on the oracle's sequence."""
+    a = _cfg1(seed=11)
+    b = pdmd.generate([8] * 6, density=0.01, tstar=0.25, seed=12)
+    eng = pdmd.Engine(waves=waves)
+    try:
+        sa = eng.add_system(a, seed=99)
+        eng.advance([sa], [6000], wait=False)
+        sb = eng.add_system(b, seed=98)
+        eng.advance([sa, sb], [2000, 5000])
+        ga, gb = eng.download(sa), eng.download(sb)
+        assert eng.stats(sa).coll == 8000 and eng.stats(sb).coll == 5000
+    finally:
+        eng.close()
+    oa = Oracle(a, seed=99, param_dir=param_dir)
+    oa.run(8000)
+    ob = Oracle(b, seed=98, param_dir=param_dir)
+    ob.run(5000)
+    for g, o in ((ga, oa.state()), (gb, ob.state())):
+        assert np.array_equal(g["sv"].view(np.int64), o["sv"].view(np.int64))
+        for k in ("identity", "bptnr", "extra", "nptnr", "coltype"):
+            assert np.array_equal(g[k], o[k]), k

@@ -1,13 +1,17 @@
 """The device engine against vectors written by the reference's own
 Fortran subroutines (tests/golden, see test_oracle_reference.py): a
 two-species system (2 chains x 8 and 2 chains x 6 residues) built plain,
-with -Dno_hbs and with -Dno_h, no thermostat.  The serial loop of eventdyn.f +
-partial_events.f + the calendar + the sync pseudo-event, as the reference's
-code ran it, is compared event by event and bit for bit with the device's
-log, and the final positions/velocities with the reference's, at 1 and 4
-wavefronts per system, and the device's checkover.f / check_nc_int.f
-kernel against the reference's own messages on the final and two planted
-states.
+with -Dno_hbs and with -Dno_h, and a dense, cool one where N-C hydrogen bonds
+form and break (the worker's resolution of main.F90:1432-1960 with every
+shoulder path), no thermostat.  The serial loop as the reference's code ran
+it is compared event by event and bit for bit with the device's log (the
+first 2e4 entries directly, the whole run through block digests), with the
+resolved-coltype tally, the capture-draw count and the final state --
+positions, velocities, calendar, identities, bptnr, extra_repuls -- at 1 and
+4 wavefronts per system; the device's checkover.f / check_nc_int.f kernel
+against the reference's own messages on the final and two planted states;
+and the device's -Drunr restore (main.F90:240-321) of the final state
+against the reference's.
 """
 import os
 
@@ -15,27 +19,34 @@ import numpy as np
 import pytest
 
 import pdmd
-from test_oracle_reference import _checks_match
+from test_oracle_reference import CASES, _checks_match, _obs_close, check_final_state, check_log
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def _fixture(name):
+    return dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
+
+
+def _sysd(fx, sv):
+    return dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=sv, boxl=float(fx["boxl"]),
+                tstar=float(fx["tstar"]))
+
+
 @pytest.mark.parametrize("waves", [1, 4])
-@pytest.mark.parametrize("name,flags", [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}),
-                                        ("ref_2species_noh", {"no_h": True})])
+@pytest.mark.parametrize("name,flags", CASES, ids=[c[0] for c in CASES])
 def test_device_matches_reference_serial_loop(name, flags, waves):
-    fx = dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
-    sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"],
-                boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
+    fx = _fixture(name)
     n = int(fx["loop_n"])
-    keep = fx["loop_i"].size
+    print(f"reference loop {name}: {n} entries at W={waves}", flush=True)
     eng = pdmd.Engine(waves=waves)
     try:
-        sid = eng.add_system(sysd, seed=1, canon=False, species=fx["species"], **flags)
-        eng.log_enable(sid, keep)
+        sid = eng.add_system(_sysd(fx, fx["sv0"]), seed=int(fx["rng_seed"]), canon=False, species=fx["species"],
+                             **flags)
+        eng.log_enable(sid, n)
         eng.advance([sid], [n])
-        g = eng.log_read(sid, keep)
+        g = eng.log_read(sid, n)
         st = eng.stats(sid)
         gs = eng.download(sid)
         # checkover.f / check_nc_int.f on the device: the final state and the
@@ -44,31 +55,47 @@ def test_device_matches_reference_serial_loop(name, flags, waves):
     finally:
         eng.close()
     assert st.status == 0 and st.coll == n
-    for k in ("i", "j", "type"):
-        bad = np.nonzero(g[k] != fx["loop_" + k])[0]
-        assert bad.size == 0, f"{k} differs first at entry {bad[0] + 1}: device {g[bad[0]]}"
-    assert np.array_equal(g["t"].view(np.int64), np.ascontiguousarray(fx["loop_t"]).view(np.int64))
-    assert np.array_equal(gs["sv"].view(np.int64), np.ascontiguousarray(fx["sv1"]).view(np.int64))
-    assert np.array_equal(gs["nptnr"], fx["nptnr1"]) and np.array_equal(gs["coltype"], fx["coltype1"])
-    assert np.array_equal(gs["tim"].view(np.int64), np.ascontiguousarray(fx["tim1"]).view(np.int64))
+    check_log(g, fx, name)
+    check_final_state(gs, fx, name)
+    assert list(st.nevents[1:31]) == [int(x) for x in fx["loop_tally"]], name
+    assert st.nupdates == fx["loop_updates"] and st.nforced == fx["loop_forced"], name
 
 
-@pytest.mark.parametrize("name,flags", [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}),
-                                        ("ref_2species_noh", {"no_h": True})])
+@pytest.mark.parametrize("name,flags", CASES, ids=[c[0] for c in CASES])
 def test_device_observables_match_reference(name, flags):
     """energy.f / radgyr.f / end2end.f on the device against the reference's
     own values before and after the serial loop (1e-12 relative: sums run
     in another order)."""
-    from test_oracle_reference import _obs_close
-
-    fx = dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
-    sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"],
-                boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
+    fx = _fixture(name)
     eng = pdmd.Engine()
     try:
-        sid = eng.add_system(sysd, seed=1, canon=False, species=fx["species"], **flags)
+        sid = eng.add_system(_sysd(fx, fx["sv0"]), seed=int(fx["rng_seed"]), canon=False, species=fx["species"],
+                             **flags)
         _obs_close(eng.observables(sid), fx["obs_f"][0], fx["obs_i"][0])
         eng.advance([sid], [int(fx["loop_n"])])
         _obs_close(eng.observables(sid), fx["obs_f"][1], fx["obs_i"][1])
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("name,flags", CASES, ids=[c[0] for c in CASES])
+def test_device_restart_restore_matches_reference(name, flags):
+    """pdmd_system_add with the final state's positions and bptnr restores
+    identities, bptnr and shoulder tables as the reference's -Drunr start-up
+    does (main.F90:240-321, restated around the reference's repuls_add), and
+    check_nc_int.f's counts on the restored state agree."""
+    fx = _fixture(name)
+    eng = pdmd.Engine()
+    try:
+        sid = eng.add_system(_sysd(fx, fx["sv1"]), seed=1, canon=False, species=fx["species"], bptnr=fx["bptnr1"],
+                             **flags)
+        gs = eng.download(sid)
+        c = eng.check(sid)
+    finally:
+        eng.close()
+    assert np.array_equal(gs["identity"], fx["rst_identity"]), name
+    assert np.array_equal(gs["bptnr"], fx["rst_bptnr"]), name
+    assert np.array_equal(gs["extra"].reshape(-1, 4), fx["rst_extra"]), name
+    _, boundbad, unboundbad, noss = (int(x) for x in fx["chk_restored_flags"])
+    assert (c["boundbad"], c["unboundbad"]) == (boundbad, unboundbad), name
+    assert (c["ss_total"] - c["ss_counted"] == 0) == (noss == 0), name

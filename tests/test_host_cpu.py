@@ -122,3 +122,27 @@ def test_header_is_plain_c99(tmp_path):
                         "-I", os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "use"),
                         "-L", pkg, "-lpdmd", "-Wl,-rpath," + pkg], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_system_bytes_counts_the_cell_grid(param_dir):
+    """pdmd_system_bytes (host-only) counts the coarse cell grid of the box
+    the system needs: (Mq^3 + 1) + Mq^3 int32 with Mq = M // 2 and M the
+    interior cells per axis of main.F90:390 (the oracle's num_cell - 2 n_wrap),
+    so a dilute box, whose grid dwarfs the per-bead buffers, costs at least
+    that much more than the same chains in a dense one."""
+    def bytes_and_grid(density):
+        sysd = pdmd.generate([8] * 4, density=density, tstar=0.2, seed=3)
+        d, keep = pdmd.Engine.make_desc(sysd, seed=1, canon=False)
+        b = pdmd.lib().pdmd_system_bytes(d)
+        assert b > 0, pdmd.lib().pdmd_last_error()
+        M = Oracle(sysd, seed=1, param_dir=param_dir, canon=False).stats().num_cell - 4
+        Mq = M // 2
+        return b, 4 * (Mq ** 3 + 1) + 4 * Mq ** 3, Mq
+
+    b_dilute, grid_dilute, mq = bytes_and_grid(1e-6)
+    b_dense, grid_dense, _ = bytes_and_grid(0.01)
+    assert mq >= 150 and grid_dilute > 20 * 2 ** 20
+    assert b_dilute >= grid_dilute
+    assert b_dilute - b_dense >= grid_dilute - grid_dense
+    # per-bead buffers: 128 beads at 64 list slots per direction
+    assert b_dense >= 128 * (64 + 2 * 64 * 4)

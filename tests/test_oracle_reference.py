@@ -3,16 +3,23 @@
 The fixtures in tests/golden/ come from oracle/_ref/ref_harness (the
 reference's unmodified Fortran compiled by oracle/ref/Makefile, driven by
 tests/golden/make_golden.py) on a two-species system (2 chains x 8 residues
-and 2 chains x 6 residues, rho* = 0.01, T* = 0.20), built plain, with
--Dno_hbs and with -Dno_h: the scaled parameter
-tables (inputinfo.f + scale_down.f), ev_code (make_code.f), the neighbour
-lists (nbor_setup.f, cell_link.f, nbor.f), the calendar after events.f,
-every pair prediction (core/bond/sqwel/sqshlder/nc_sqwel.f), the energy and
-shape observables (energy.f, radgyr.f, end2end.f) and a serial run of
-eventdyn.f + partial_events.f + the calendar + the sync pseudo-event in the
-reference's order.  Everything is compared bit for bit, observables to
-1e-12 relative (their sums run in a different order).
+and 2 chains x 6 residues): rho* = 0.01, T* = 0.20, built plain, with
+-Dno_hbs and with -Dno_h, and rho* = 0.30, T* = 0.18 plain, where N-C
+hydrogen bonds form and break.  They hold the scaled parameter tables
+(inputinfo.f + scale_down.f), ev_code (make_code.f), the neighbour lists
+(nbor_setup.f, cell_link.f, nbor.f), the calendar after events.f, every pair
+prediction (core/bond/sqwel/sqshlder/nc_sqwel.f), the energy and shape
+observables (energy.f, radgyr.f, end2end.f), a serial run of the event loop
+in the reference's order -- every pair event through the worker's resolution
+of main.F90:1432-1960 around the reference's repuls_check, repuls_check_3,
+check_sigma, eventdyn, bumpoff, repuls_add, repuls_del_a/b and
+partial_events, the calendar and the sync and output pseudo-events -- with
+the resolved-coltype tally, the final state, check_nc_int.f and checkover.f
+on it, and the -Drunr restore of that state (main.F90:240-321) with
+check_nc_int.f on the result.  Everything is compared bit for bit,
+observables to 1e-12 relative (their sums run in a different order).
 """
+import hashlib
 import os
 
 import numpy as np
@@ -22,17 +29,56 @@ from oracle import Oracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # (fixture, build flags of the reference harness that wrote it)
-CASES = [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}), ("ref_2species_noh", {"no_h": True})]
+CASES = [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}), ("ref_2species_noh", {"no_h": True}),
+         ("ref_2species_hb", {})]
+# coltypes of the worker's N-C resolution (main.F90:1487-1937, eventdyn.f)
+HB_TYPES = (14, 15, 16, 23, 24, 25, 26)
+LOG_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("t", "<f8")])
+
+
+def block_digests(log, block=1000):
+    """8-byte BLAKE2b digests of consecutive blocks of log entries packed as
+    tests/golden/make_golden.py packs the reference's."""
+    rec = np.zeros(len(log), LOG_DTYPE)
+    for k in ("i", "j", "type", "t"):
+        rec[k] = log[k]
+    raw = rec.tobytes()
+    step = block * LOG_DTYPE.itemsize
+    return np.array([np.frombuffer(hashlib.blake2b(raw[k:k + step], digest_size=8).digest(), "<u8")[0]
+                     for k in range(0, len(raw), step)], np.uint64)
+
+
+def check_log(log, fx, name):
+    """Event by event over the kept window, block digests over the rest."""
+    keep = fx["loop_i"].size
+    for k in ("i", "j", "type"):
+        bad = np.nonzero(log[k][:keep] != fx["loop_" + k])[0]
+        assert bad.size == 0, f"{name}: {k} differs first at entry {bad[0] + 1}: {log[bad[0]]}"
+    assert np.array_equal(_bits(log["t"][:keep]), _bits(fx["loop_t"])), name
+    dg = block_digests(log)
+    bad = np.nonzero(dg != fx["loop_digest"])[0]
+    assert dg.size == fx["loop_digest"].size and bad.size == 0, \
+        f"{name}: log differs in block {bad[0] if bad.size else '?'} of {fx['loop_digest'].size}"
+
+
+def check_final_state(st, fx, name):
+    assert np.array_equal(_bits(st["sv"]), _bits(fx["sv1"])), name
+    assert np.array_equal(_bits(st["tim"]), _bits(fx["tim1"])), name
+    assert np.array_equal(st["nptnr"], fx["nptnr1"]), name
+    assert np.array_equal(st["coltype"], fx["coltype1"]), name
+    assert np.array_equal(st["identity"], fx["identity1"]), name
+    assert np.array_equal(st["bptnr"], fx["bptnr1"]), name
+    assert np.array_equal(np.asarray(st["extra"]).reshape(-1, 4), fx["extra1"]), name
 
 
 def _load(name):
     return dict(np.load(os.path.join(HERE, "golden", name + ".npz")))
 
 
-def _oracle(fx, param_dir, flags):
+def _oracle(fx, param_dir, flags, seed=1):
     sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"],
                 boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
-    return Oracle(sysd, seed=1, param_dir=param_dir, canon=False, species=fx["species"], **flags)
+    return Oracle(sysd, seed=seed, param_dir=param_dir, canon=False, species=fx["species"], **flags)
 
 
 def _bits(a):
@@ -130,24 +176,59 @@ def test_serial_loop(param_dir):
     # fresh oracles: the module fixture's ones have been queried only
     for name, flags in CASES:
         fx = _load(name)
-        orc = _oracle(fx, param_dir, flags)
+        orc = _oracle(fx, param_dir, flags, seed=int(fx["rng_seed"]))
         n = int(fx["loop_n"])
-        keep = fx["loop_i"].size
-        log = orc.run(n, log_cap=keep)
-        for k in ("i", "j", "type"):
-            bad = np.nonzero(log[k] != fx["loop_" + k])[0]
-            assert bad.size == 0, f"{name}: {k} differs first at entry {bad[0] + 1}"
-        assert np.array_equal(_bits(log["t"]), _bits(fx["loop_t"])), name
-        st = orc.state()
-        assert np.array_equal(_bits(st["sv"]), _bits(fx["sv1"])), name
-        assert np.array_equal(_bits(st["tim"]), _bits(fx["tim1"])), name
-        assert np.array_equal(st["nptnr"], fx["nptnr1"]), name
-        assert np.array_equal(st["coltype"], fx["coltype1"]), name
+        log = orc.run(n, log_cap=n)
+        check_log(log, fx, name)
+        check_final_state(orc.state(), fx, name)
         s = orc.stats()
         assert s.nupdates == fx["loop_updates"] and s.nforced == fx["loop_forced"], name
         assert s.tfalse == fx["tfalse1"], name
+        assert list(s.nevents[1:31]) == [int(x) for x in fx["loop_tally"]], name
+        assert s.rng_ctr[1] == fx["loop_draws"], name
         _obs_close(orc.observables(), fx["obs_f"][1], fx["obs_i"][1])
         _checks_match(orc.check, fx, name)
+
+
+def test_hbond_fixture_exercises_the_resolution():
+    """The H-bond fixture's loop runs every branch of the worker's N-C
+    resolution: captures refused (14), shoulder exits/entries (15, 24, 25,
+    26), wells left with shoulders removed (16), end-bead capture draws (23
+    and the draw count), and bonds stand at the end."""
+    fx = _load("ref_2species_hb")
+    tally = fx["loop_tally"]
+    for ty in HB_TYPES:
+        assert tally[ty - 1] > 0, f"no coltype {ty} in the H-bond fixture"
+    assert sum(int(tally[ty - 1]) for ty in (20, 21) + HB_TYPES) >= 300
+    assert int(fx["loop_draws"]) > 0
+    assert (fx["bptnr1"] > 0).sum() >= 2
+    assert (fx["identity1"] >= 5).sum() >= 2 and (fx["identity1"] <= 8).sum() > 0
+
+
+def test_restart_restore(param_dir):
+    """-Drunr restore (main.F90:240-321) of each fixture's final state:
+    identities, bptnr, extra_repuls and ev_code as the harness's restatement
+    around the reference's repuls_add left them, and check_nc_int.f's counts
+    on the restored state."""
+    for name, flags in CASES:
+        fx = _load(name)
+        sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv1"],
+                    boxl=float(fx["boxl"]), tstar=float(fx["tstar"]))
+        orc = Oracle(sysd, seed=1, param_dir=param_dir, canon=False, species=fx["species"], bptnr=fx["bptnr1"],
+                     **flags)
+        st = orc.state()
+        assert np.array_equal(st["identity"], fx["rst_identity"]), name
+        assert np.array_equal(st["bptnr"], fx["rst_bptnr"]), name
+        assert np.array_equal(np.asarray(st["extra"]).reshape(-1, 4), fx["rst_extra"]), name
+        N = orc.N
+        got = np.array([[orc.ev_code(a, b) for b in range(1, N + 1)] for a in range(1, N + 1)], np.int32)
+        assert np.array_equal(got, fx["rst_ev_code"]), name
+        _, boundbad, unboundbad, noss = (int(x) for x in fx["chk_restored_flags"])
+        c = orc.check()
+        assert (c["boundbad"], c["unboundbad"]) == (boundbad, unboundbad), name
+        assert (c["ss_total"] - c["ss_counted"] == 0) == (noss == 0), name
+    fx = _load("ref_2species_hb")
+    assert (fx["rst_extra"][:, 3] > 0).sum() >= 2 and (fx["rst_ev_code"] >= 40).sum() >= 8
 
 
 def planted_states(fx):
