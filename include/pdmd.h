@@ -199,6 +199,14 @@ int pdmd_pair_codes(const pdmd_system_desc* desc, int32_t n, const int32_t* a, c
  * resident state at the current tfalse: side-chain well energies over the
  * neighbour lists instead of energy.f's all-pairs loop. */
 int pdmd_observables(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
+/* The same observables, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): no counterpart in the reference.  Identical results; nothing
+ * is allocated per call -- the chain tables and results pass through one
+ * fixed 33 KB device array per GPU, located at context creation, so systems
+ * of at most 4096 chains (every system pdmd_system_add accepts).  Calls
+ * that overlap on one GPU, from any context, race on that array. */
+int pdmd_observables_noalloc(pdmd_ctx* ctx, int32_t sys_id, pdmd_obs_t* out);
 
 typedef struct {
     int64_t overlaps;    /* checkover.f:21-131: pairs with ev_code 1 or >= 15 closer than sigma */

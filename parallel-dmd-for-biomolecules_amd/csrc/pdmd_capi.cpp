@@ -459,6 +459,8 @@ int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
             ctx->lds_limit = (size_t)std::max(lds, 32768);
         }
     }
+    // the fixed staging array of pdmd_observables_noalloc on this device
+    HIPCHK(engine_scratch_init(ctx->device));
     {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
@@ -1033,6 +1035,32 @@ int pdmd_observables(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
     int n[3];
     HIPCHK(hipMemcpy(d, od.p, sizeof d, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(n, oi.p, sizeof n, hipMemcpyDeviceToHost));
+    std::memset(o, 0, sizeof *o);
+    o->ered = d[0];
+    o->tred = d[1];
+    o->sumvel = d[2];
+    o->ehh_ii = d[3];
+    o->ehh_ij = d[4];
+    o->rg_avg = d[5];
+    o->e2e_avg = d[6];
+    o->hb_alpha = n[0];
+    o->hb_ii = n[1];
+    o->hb_ij = n[2];
+    return 0;
+}
+
+int pdmd_observables_noalloc(pdmd_ctx* ctx, int32_t id, pdmd_obs_t* o) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H || !o) return fail("pdmd_observables_noalloc: bad arguments");
+    const Topology& t = H->topo;
+    const int nch = (int)t.chain_len.size();
+    if (nch > 4096) return fail("pdmd_observables_noalloc: at most 4096 chains (the staging array's size)");
+    int nfirst = 0;  // chains of the leading species block: the reference's species 1
+    while (nfirst < nch && t.species[nfirst] == t.species[0]) nfirst++;
+    double d[7];
+    int n[3];
+    HIPCHK(engine_observables_noalloc(ctx->d_sys.p, id - 1, ctx->device, t.chain_start.data(), t.chain_len.data(), nch,
+                                      nfirst, H->p.boxl_orig, d, n, ctx->stream));
     std::memset(o, 0, sizeof *o);
     o->ered = d[0];
     o->tred = d[1];

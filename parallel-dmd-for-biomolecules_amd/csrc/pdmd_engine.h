@@ -41,6 +41,15 @@ hipError_t engine_check(const SysDev* d_sys, int id, int n, const Rec* alt, unsi
 // nfirst: chains of the leading species block; boxl in Angstrom.
 hipError_t engine_observables(const SysDev* d_sys, int id, const int* cstart, const int* clen, int nch, int nfirst,
                               double boxl, double* outd, int* outi, hipStream_t s);
+// The same without device allocation (pdmd_observables_noalloc): the chain
+// tables (host arrays, nch <= 4096) and the results pass through a fixed
+// 33 KB device array per GPU; returns after the results reached h_outd[7]
+// and h_outi[3].  Calls that overlap on one GPU race on that array.
+hipError_t engine_observables_noalloc(const SysDev* d_sys, int id, int device, const int* h_cstart, const int* h_clen,
+                                      int nch, int nfirst, double boxl, double* h_outd, int* h_outi, hipStream_t s);
+// Looks up this GPU's copy of that array (device must be current); done
+// once per device at context creation.
+hipError_t engine_scratch_init(int device);
 // Test hook: collision times of (i, j) pairs (1-based) of system id.
 hipError_t engine_predict(const SysDev* d_sys, int id, int n, const int* pi, const int* pj, double* tij, int* type,
                           hipStream_t s);

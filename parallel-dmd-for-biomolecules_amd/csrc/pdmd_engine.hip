@@ -22,6 +22,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <mutex>
 
 #include "pdmd_device.h"
 #include "pdmd_engine.h"
@@ -3913,6 +3914,67 @@ hipError_t engine_observables(const SysDev* d_sys, int id, const int* cstart, co
     hipLaunchKernelGGL(k_obs, dim3(1), dim3(kObsT), 0, s, d_sys, id, cstart, clen, nch, nfirst, boxl, outd, outi);
     return hipGetLastError();
 }
+// ------------------------------------------------------------------------
+// observables without device allocation: the chain tables and the results
+// are staged in one fixed device array per GPU
+// ------------------------------------------------------------------------
+constexpr int kScratchChains = 4096;  // a system holds at most 4095 chains
+constexpr size_t kScratchClen = sizeof(int) * kScratchChains;
+constexpr size_t kScratchOutd = 2 * kScratchClen;
+constexpr size_t kScratchOuti = kScratchOutd + 64;
+constexpr size_t kScratchBytes = kScratchOuti + 256;  // 33 024 bytes
+// Host code must not use this name as a device address: each GPU holds its
+// own copy, whose address engine_scratch_init looks up.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices];
+}  // namespace
+
+hipError_t engine_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<unsigned char*>(p);
+    return hipSuccess;
+}
+
+hipError_t engine_observables_noalloc(const SysDev* d_sys, int id, int device, const int* h_cstart, const int* h_clen,
+                                      int nch, int nfirst, double boxl, double* h_outd, int* h_outi, hipStream_t s) {
+    if (nch < 1 || nch > kScratchChains) return hipErrorInvalidValue;
+    hipError_t e = engine_scratch_init(device);
+    if (e != hipSuccess) return e;
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch[device];
+    }
+    int* cs = reinterpret_cast<int*>(base);
+    int* cl = reinterpret_cast<int*>(base + kScratchClen);
+    double* od = reinterpret_cast<double*>(base + kScratchOutd);
+    int* oi = reinterpret_cast<int*>(base + kScratchOuti);
+    e = hipMemcpyAsync(cs, h_cstart, sizeof(int) * (size_t)nch, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(cl, h_clen, sizeof(int) * (size_t)nch, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_obs, dim3(1), dim3(kObsT), 0, s, d_sys, id, cs, cl, nch, nfirst, boxl, od, oi);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(h_outd, od, sizeof(double) * 7, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(h_outi, oi, sizeof(int) * 3, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
 hipError_t engine_prepare(size_t lds_limit) {
     hipError_t e = hipFuncSetAttribute((const void*)k_run, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
     if (e != hipSuccess) return e;

@@ -132,6 +132,7 @@ EXPORTS = [
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
     "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
     "pdmd_set_waves", "pdmd_autotune_waves", "pdmd_check", "pdmd_set_graph_launch", "pdmd_rebuild",
+    "pdmd_observables_noalloc",
 ]
 
 _lib = None
@@ -170,6 +171,7 @@ def lib():
         L.pdmd_log_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, _i32p]
         L.pdmd_predict_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f64p, _i32p]
         L.pdmd_observables.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
+        L.pdmd_observables_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
         L.pdmd_generate.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
         L.pdmd_server_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
@@ -372,6 +374,12 @@ class Engine:
     def observables(self, sid):
         o = Obs()
         _check(lib().pdmd_observables(self._ctx, sid, ctypes.byref(o)))
+        return o
+
+    def observables_noalloc(self, sid):
+        """pdmd_observables_noalloc: the same values, no device allocation."""
+        o = Obs()
+        _check(lib().pdmd_observables_noalloc(self._ctx, sid, ctypes.byref(o)))
         return o
 
 
