@@ -119,7 +119,8 @@ int pdmd_advance(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int6
 int pdmd_advance_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* nevents);
 int pdmd_wait(pdmd_ctx* ctx);
 /* The same with a device wall-clock budget: each system stops between
- * events once budget_ms have passed since the launch began, or at
+ * events once budget_ms have passed since the launch began (the first of
+ * its workgroups started), or at
  * max_nevents[k] more entries, whichever comes first (pdmd_stats gives the
  * entries committed).  For request loops over systems of uneven speed. */
 int pdmd_advance_budget_async(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, const int64_t* max_nevents,

@@ -70,6 +70,7 @@ struct HostSys {
     DevBuf<double4> old_r, spos;
     DevBuf<int4> sinfo;
     DevBuf<unsigned> mark;
+    DevBuf<unsigned long long> stamp;
     DevBuf<SysState> st;
     DevBuf<LogRec> log;
     SysDev dev{};
@@ -86,6 +87,7 @@ struct pdmd_ctx {
     pdmd::DevBuf<pdmd::SysDev> d_sys;
     pdmd::DevBuf<int> d_ids;
     pdmd::DevBuf<long long> d_inc;
+    pdmd::DevBuf<unsigned long long> d_clock;  // start of a budgeted launch
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -399,7 +401,8 @@ int64_t device_bytes(const Prep& pp) {
                              N * 16 /*ovr*/, N * 32 /*old_r*/, N * cap * 4 /*nb_up*/, N * cap * 4 /*nb_dn*/,
                              N * 4 /*nup*/, N * 4 /*ndn*/, (N + 3) * (int64_t)sizeof(Leaf), N * 4 /*cfine*/,
                              N * 4 /*ckey*/, N * 4 /*csorted*/, N * 32 /*spos*/, N * 16 /*sinfo*/, N * 4 /*mark*/,
-                             (nc + 1) * 4 /*cstart*/, nc * 4 /*crun*/, (int64_t)sizeof(SysState)};
+                             (nc + 1) * 4 /*cstart*/, nc * 4 /*crun*/, (N + 3) * 8 /*stamp*/,
+                             (int64_t)sizeof(SysState)};
     int64_t b = 0;
     for (int64_t x : sizes) b += (x + 4095) / 4096 * 4096;
     return b;
@@ -461,6 +464,7 @@ int pdmd_ctx_create(const char* param_dir, pdmd_ctx** out) {
     }
     // the fixed staging array of pdmd_observables_noalloc on this device
     HIPCHK(engine_scratch_init(ctx->device));
+    HIPCHK(ctx->d_clock.alloc(1));
     {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && khz > 0)
@@ -537,6 +541,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     HIPCHK(H.sinfo.alloc(N));
     HIPCHK(H.mark.alloc(N));
     HIPCHK(hipMemsetAsync(H.mark.p, 0, sizeof(unsigned) * (size_t)N, ctx->stream));
+    HIPCHK(H.stamp.alloc(N + 3));
+    HIPCHK(hipMemsetAsync(H.stamp.p, 0, sizeof(unsigned long long) * (size_t)(N + 3), ctx->stream));
     HIPCHK(H.cstart.alloc((size_t)Mq * Mq * Mq + 1));
     HIPCHK(H.crun.alloc((size_t)Mq * Mq * Mq));
     {
@@ -577,6 +583,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.spos = H.spos.p;
     D.sinfo = H.sinfo.p;
     D.mark = H.mark.p;
+    D.stamp = H.stamp.p;
     D.st = H.st.p;
     D.log = nullptr;
     H.alive = true;
@@ -687,7 +694,7 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
     const int maxw = engine_maxw(max_cap);
     if (!ctx->graphs) {
         HIPCHK(engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, maxw, ctx->lds_limit, ctx->stream,
-                          ticks, flags, ctx->waves));
+                          ticks, flags, ctx->waves, ctx->d_clock.p));
     } else {
         // the system list and event targets live in device buffers, so one
         // captured launch serves every step of the same shape
@@ -701,7 +708,7 @@ int advance_launch(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, const int64_
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
             const hipError_t er = engine_run(ctx->d_sys.p, ctx->d_ids.p, ctx->d_inc.p, nsys, max_n, maxw,
-                                             ctx->lds_limit, ctx->stream, ticks, flags, ctx->waves);
+                                             ctx->lds_limit, ctx->stream, ticks, flags, ctx->waves, ctx->d_clock.p);
             const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
             const hipError_t ei = er == hipSuccess && ec == hipSuccess ? hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0)
                                                                        : hipSuccess;

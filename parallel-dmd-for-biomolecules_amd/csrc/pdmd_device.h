@@ -112,6 +112,11 @@ struct SysDev {
     double4* spos; // [N] positions in coarse-cell order (list rebuild)
     int4* sinfo;   // [N] bead, topo word, fine cell, 0 in coarse-cell order
     unsigned* mark;  // [N] per-round footprint marks of a multi-wavefront system (zero between rounds)
+    // [N+3] when each calendar entry was last added (add_tbin.f): the
+    // committed-entry count of the event that added it << 20 | the add's
+    // place within that event; of two entries with equal bin and time the
+    // later-added one comes first, as in the reference's LIFO bucket lists
+    unsigned long long* stamp;
     SysState* st;
     LogRec* log;
 };

@@ -24,13 +24,15 @@ hipError_t engine_setup(const SysDev* d_sys, const int* d_ids, int n, int max_n,
 hipError_t engine_rebuild(const SysDev* d_sys, const int* d_ids, int n, int max_n, int maxd, size_t lds_limit,
                           hipStream_t s);
 // Event loop: every listed system runs until its SysState::target.
-// budget_ticks: wall_clock64 ticks after which each system stops between
-// rounds (0 = run to the event targets); waves: wavefronts per system (1 to
-// 8, lowered to what fits in lds_limit), each round runs up to that many
-// conflict-free pair events side by side
+// budget_ticks: wall_clock64 ticks after the launch's start after which
+// each system stops between rounds (0 = run to the event targets), with
+// launch_t0 a device word that takes the start (needed with a budget);
+// waves: wavefronts per system (1 to 8, lowered to what fits in lds_limit),
+// each round runs up to that many conflict-free pair events side by side
 constexpr int kStopAtOutput = 1;  // engine_run flag: stop after the next output pseudo-event
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s, long long budget_ticks = 0, int flags = 0, int waves = 1);
+                      size_t lds_limit, hipStream_t s, long long budget_ticks = 0, int flags = 0, int waves = 1,
+                      unsigned long long* launch_t0 = nullptr);
 // Debug checks (checkover.f, check_nc_int.f) of system id's n beads: out[8]
 // counters (zeroed by the caller, out[3] preset to all ones); alt replaces
 // the resident state records when not null.

@@ -485,19 +485,48 @@ __device__ inline void wave_min_pred(double& t, int& p, int& m, int& ty) {
     }
 }
 
-// Calendar key of one entry: order (bin, time, index), payload partner and
-// type.  bc = bin << 8 | (coltype + 2); ip = index << 16 | (partner + 2).
+// Calendar key of one entry: order (bin, time, later add first), payload
+// partner and type.  bc = bin << 8 | (coltype + 2); ip = index << 16 |
+// (partner + 2).  Equal bins and times, which only bit-identical event
+// times give, are settled by the entries' add stamps: the reference's
+// bucket lists put each add at the head (add_tbin.f:28-32) and its stable
+// sort keeps list order among equal times (main.F90:521-545), so the entry
+// added last is taken first.
 struct Key {
     double t;
     int bc;
     unsigned ip;
 };
 constexpr int kEmptyBc = INT_MAX;
+constexpr int kStampBits = 20;  // adds per event: up to N + 4 in a sync's list rebuild
+__device__ inline unsigned long long mk_stamp(long long coll, int sub) {
+    return ((unsigned long long)coll << kStampBits) | (unsigned long long)(unsigned)sub;
+}
+// stamps live in HBM and are read and written at the device's L2 (other
+// wavefronts of the workgroup write them)
+__device__ inline void stamp_set(int k, unsigned long long v) {
+    __hip_atomic_store((GLB unsigned long long*)(sdev().stamp + k), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long stamp_get(int k) {
+    return __hip_atomic_load((GLB unsigned long long*)(sdev().stamp + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a before b in the order above, with a's and b's stamps given (entries not
+// yet written: a round's new keys)
+__device__ inline bool key_less_st(const Key& a, unsigned long long sa, const Key& b, unsigned long long sb) {
+    const int ab = a.bc >> 8, bb = b.bc >> 8;
+    if (ab != bb) return ab < bb;
+    if (a.t != b.t) return a.t < b.t;
+    if (a.bc == kEmptyBc || sa == sb) return a.ip < b.ip;
+    return sa > sb;
+}
+// a before b, both entries of the calendar (their stamps written)
 __device__ inline bool key_less(const Key& a, const Key& b) {
     const int ab = a.bc >> 8, bb = b.bc >> 8;
     if (ab != bb) return ab < bb;
     if (a.t != b.t) return a.t < b.t;
-    return a.ip < b.ip;
+    if (a.bc == kEmptyBc || a.ip == b.ip) return a.ip < b.ip;
+    const unsigned long long sa = stamp_get((int)(a.ip >> 16)), sb = stamp_get((int)(b.ip >> 16));
+    return sa != sb ? sa > sb : a.ip < b.ip;
 }
 __device__ inline Key empty_key() {
     Key e;
@@ -608,6 +637,7 @@ __device__ inline unsigned long long t_order(double t) {
 struct Batch {
     Key key[kMaxWaves];   // popped entries
     Key knew[kMaxWaves];  // least key among the wavefront's new calendar entries
+    unsigned long long knew_st[kMaxWaves];  // its add stamp
     int ty[kMaxWaves];    // resolved type of each event
     int nbl[kMaxWaves];   // blocks to rescan, in each wavefront's list
     int st[kMaxWaves];    // status after each event
@@ -616,6 +646,7 @@ struct Batch {
     int cut;              // first wavefront whose event is not committed
     int rebuild;          // the round's sync event rebuilds the lists
     int gbead;            // the bead of a ghost event leading the round
+    long long t0;         // start of a budgeted launch (wall_clock64)
 };
 // Order proxy of a calendar key: the bin in the top 12 bits, then the top
 // 52 bits of the ordered time.  Proxies order keys as key_less does except
@@ -656,6 +687,41 @@ __device__ inline Key key_min_wave(Key k) {
         rest &= rest - 1;
         if (key_less(o, r)) r = o;
     }
+    return r;
+}
+// minimum over the wave of keys with their add stamps given per lane (new
+// entries, not yet written), returned to every lane with its stamp
+__device__ inline Key key_min_wave_st(Key k, unsigned long long st, unsigned long long& out_st) {
+    const unsigned long long p = key_proxy(k);
+    unsigned long long m = p;
+    umin_step<0x111, 0xf>(m);
+    umin_step<0x112, 0xf>(m);
+    umin_step<0x114, 0xf>(m);
+    umin_step<0x118, 0xf>(m);
+    umin_step<0x142, 0xa>(m);
+    umin_step<0x143, 0xc>(m);
+    const unsigned long long mn = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)m, 63) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(m >> 32), 63) << 32);
+    out_st = 0;
+    if (mn == ~0ull) return empty_key();
+    const unsigned long long tie = __ballot(p == mn);
+    int src = __ffsll((long long)tie) - 1;
+    Key r = readlane_key(k, src);
+    unsigned long long rs = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)st, src) |
+                            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(st >> 32), src) << 32);
+    unsigned long long rest = tie & (tie - 1);
+    while (rest) {
+        src = __ffsll((long long)rest) - 1;
+        rest &= rest - 1;
+        const Key o = readlane_key(k, src);
+        const unsigned long long os = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)st, src) |
+                                      ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(st >> 32), src) << 32);
+        if (key_less_st(o, os, r, rs)) {
+            r = o;
+            rs = os;
+        }
+    }
+    out_st = rs;
     return r;
 }
 // Minimum over each row of 16 lanes.  Returns true on the lane that holds
@@ -921,11 +987,18 @@ __device__ inline void round_clear_marks(const SysDev& S, int dent, int i, int j
 __device__ inline int batch_accepted(const Batch& B) {
     int A = min(B.K, B.cut);
     Key lo = B.knew[0];
+    unsigned long long lo_st = B.knew_st[0];
     if (B.st[0] != 0) A = min(A, 1);
     for (int k = 1; k < A; ++k) {
-        if (!key_less(B.key[k], lo)) return k;
+        const Key kk = B.key[k];
+        // kk was added before the round began, lo during it: on equal bin
+        // and time lo comes first (stamp 0 is older than every add)
+        if (!key_less_st(kk, 0ull, lo, lo_st)) return k;
         if (B.st[k] != 0) return k + 1;
-        if (key_less(B.knew[k], lo)) lo = B.knew[k];
+        if (key_less_st(B.knew[k], B.knew_st[k], lo, lo_st)) {
+            lo = B.knew[k];
+            lo_st = B.knew_st[k];
+        }
     }
     return A;
 }
@@ -1272,6 +1345,7 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
             lf.nptnr = st_p;
             lf.meta = make_meta(calc_bin(c, st_t, false, status), st_c);
             leaf_set(L, l, lf);
+            stamp_set(l, mk_stamp(c.coll, (j >= 0 ? 3 : 4) + q));
         }
         const unsigned long long mm = __ballot(modified);
         const int pre = __popcll(mm & ((1ull << lane) - 1ull));
@@ -1281,17 +1355,21 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
     }
     PH_END(S, kPhDown, tph);
     if (lane == 0) {
+        // adds in partial_events.f order: i, j, then the down set (a ghost
+        // event's own entry came first)
         Leaf a;
         a.tim = ti;
         a.nptnr = pi;
         a.meta = make_meta(calc_bin(c, ti, false, status), ci);
         leaf_set(L, i, a);
+        stamp_set(i, mk_stamp(c.coll, j >= 0 ? 1 : 2));
         if (j >= 0) {
             Leaf b;
             b.tim = tj;
             b.nptnr = pj;
             b.meta = make_meta(calc_bin(c, tj, false, status), cj);
             leaf_set(L, j, b);
+            stamp_set(j, mk_stamp(c.coll, 2));
         }
     }
     // repuls_del_b.f once the rescheduling is done
@@ -1363,6 +1441,9 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const double T0 = c.interval_max + kLt - tf;
     const int dbase = maxw - (2 * cap + 8);  // down-set region of the work list
     const bool batch = bk >= 0;
+    // place of i's add among this event's calendar adds (a ghost event adds
+    // its own entry first); j follows, then the down set in list order
+    const int sub0 = j >= 0 ? 1 : 2;
     int status = 0, nb = 0;
     PH_BEGIN(tph);
     // ---- first loads: list rows of i and j, whole rows with their lengths
@@ -1862,21 +1943,36 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         Batch& B = *L.batch();
         PH_END(S, kPhDown, tph);
         Key kn = wr ? leaf_key(lf, l) : empty_key();
+        unsigned long long ks = wr ? mk_stamp(c.coll, sub0 + 2 + lane) : 0ull;
         if (lane == 0) {
             const Key ka = leaf_key(la, i);
-            if (key_less(ka, kn)) kn = ka;
+            const unsigned long long sa = mk_stamp(c.coll, sub0);
+            if (key_less_st(ka, sa, kn, ks)) {
+                kn = ka;
+                ks = sa;
+            }
             if (j >= 0) {
                 const Key kb = leaf_key(lb, j);
-                if (key_less(kb, kn)) kn = kb;
+                const unsigned long long sb = mk_stamp(c.coll, sub0 + 1);
+                if (key_less_st(kb, sb, kn, ks)) {
+                    kn = kb;
+                    ks = sb;
+                }
             } else {
                 // a ghost event leading the round: its next ghost time
                 const Key kg = leaf_key(leaf_get(L, S.N), S.N);
-                if (key_less(kg, kn)) kn = kg;
+                const unsigned long long sg = mk_stamp(c.coll, 1);  // re-added by this ghost event
+                if (key_less_st(kg, sg, kn, ks)) {
+                    kn = kg;
+                    ks = sg;
+                }
             }
         }
-        kn = key_min_wave(kn);
+        unsigned long long kns = 0;
+        kn = key_min_wave_st(kn, ks, kns);
         if (lane == 0) {
             B.knew[bk] = kn;
+            B.knew_st[bk] = kns;
             B.st[bk] = c.status;
         }
         bar_to(nb, kRoundBars);
@@ -1925,12 +2021,14 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             nl.nptnr = rp;
             nl.meta = make_meta(calc_bin(c, rt, false, st2), rc);
             leaf_set(L, l, nl);
+            stamp_set(l, mk_stamp(c.coll, sub0 + 2 + lane));
         }
         st2 = wave_status(st2);
         if (st2 && lane == 0 && B.st[bk] == 0) B.st[bk] = st2;
         wb = mod;
     } else if (wr) {
         leaf_set(L, l, lf);
+        stamp_set(l, mk_stamp(c.coll, sub0 + 2 + lane));
     }
     if (lane == 0) {
         L.blist()[0] = i / kBlk;
@@ -1944,7 +2042,11 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     PH_END(S, kPhDown, tph);
     if (lane == 0) {
         leaf_set(L, i, la);
-        if (j >= 0) leaf_set(L, j, lb);
+        stamp_set(i, mk_stamp(c.coll, sub0));
+        if (j >= 0) {
+            leaf_set(L, j, lb);
+            stamp_set(j, mk_stamp(c.coll, sub0 + 1));
+        }
     }
     // repuls_del_b.f once the rescheduling is done
     if (xpulse_del && lane == 0) {
@@ -2457,6 +2559,7 @@ __device__ int ghost_event(const SysDev&, const DevTables&, const Lds&, Sc&, int
         g.nptnr = -2;
         g.meta = make_meta(calc_bin(c, tg, false, status), -2);
         leaf_set(L, N, g);
+        stamp_set(N, mk_stamp(c.coll, 1));  // added before partial_events' adds (main.F90:1041-1049)
     }
     if (status && !c.status) c.status = status;
     if (c.tfalse < c.old_tfalse) c.tfalse = c.old_tfalse;
@@ -2900,11 +3003,13 @@ __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
         lf.nptnr = pi;
         lf.meta = make_meta(calc_bin(c, ti, false, status), ci);
         leaf_set(L, i, lf);
+        stamp_set(i, mk_stamp(c.coll, i + 1));  // events.f:111-122 adds in index order
     }
     for (int e = N + tid; e < N + 3; e += nthr) {
         Leaf lf = leaf_get(L, e);
         lf.meta = make_meta(calc_bin(c, lf.tim, false, status), -2);
         leaf_set(L, e, lf);
+        stamp_set(e, mk_stamp(c.coll, e + 1));
     }
     status = wave_status(status);
     if (status && !c.status) c.status = status;
@@ -3035,6 +3140,7 @@ __device__ void sync_finish(const SysDev&, const Lds&, Sc&) {
         lf.nptnr = -2;
         lf.meta = make_meta(calc_bin(c, t2, true, status), -2);
         leaf_set(L, N + 1, lf);
+        stamp_set(N + 1, mk_stamp(c.coll, N + 4));  // after a rebuild's events() adds
         L.blist()[0] = (N + 1) / kBlk;
     }
     if (status && !c.status) c.status = status;
@@ -3063,6 +3169,7 @@ __device__ void output_event(const SysDev&, const Lds&, Sc&) {
         lf.nptnr = -2;
         lf.meta = make_meta(calc_bin(c, t3, false, status), -2);
         leaf_set(L, N + 2, lf);
+        stamp_set(N + 2, mk_stamp(c.coll, 1));
         L.blist()[0] = (N + 2) / kBlk;
     }
     if (status && !c.status) c.status = status;
@@ -3401,7 +3508,8 @@ __device__ inline void log_event(const SysDev& S, int& log_count, long long coll
 // evenly; flags & kStopAtOutput stops it right after its next output
 // pseudo-event (where the reference writes observables and configurations).
 __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict__ sys, const int* ids, const long long* inc,
-                                                       int maxw, int lds_bytes, long long budget, int flags) {
+                                                       int maxw, int lds_bytes, long long budget, int flags,
+                                                       unsigned long long* launch_t0) {
     const int nw = (int)(blockDim.x >> 6), wave = wave_id(), lane = lane_id();
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
@@ -3425,7 +3533,18 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
     __syncthreads();
     rescan_all(L, N3);
     int log_count = S.st->log_count;
-    const long long w0 = wall_clock64();
+    // the budget runs from the launch's start: the earliest start among the
+    // workgroups (launch_t0 is reset to all ones before a budgeted launch),
+    // so that a workgroup dispatched late does not get a later deadline
+    long long w0 = wall_clock64();
+    if (budget > 0) {
+        if (threadIdx.x == 0) {
+            atomicMin(launch_t0, (unsigned long long)w0);
+            B.t0 = (long long)__hip_atomic_load(launch_t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        w0 = B.t0;
+    }
     bool stop_out = false;
     while (true) {
         if (wave == 0) {
@@ -3868,8 +3987,13 @@ hipError_t engine_rebuild(const SysDev* d_sys, const int* d_ids, int n, int max_
     hipLaunchKernelGGL(k_rebuild, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
     return hipGetLastError();
 }
+// clears the start word of a budgeted launch (a kernel rather than a memset,
+// so that a captured graph replays it faithfully)
+__global__ void k_clock_reset(unsigned long long* w) { *w = ~0ull; }
+
 hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_inc, int n, int max_n, int maxw,
-                      size_t lds_limit, hipStream_t s, long long budget_ticks, int flags, int waves) {
+                      size_t lds_limit, hipStream_t s, long long budget_ticks, int flags, int waves,
+                      unsigned long long* launch_t0) {
     const int nblk = nblk_for(max_n);
     int nw = waves < 1 ? 1 : (waves > kMaxWaves ? kMaxWaves : waves);
     while (nw > 1 && engine_lds_bytes(nblk, maxw, 0, nw) > lds_limit) --nw;
@@ -3892,7 +4016,12 @@ hipError_t engine_run(const SysDev* d_sys, const int* d_ids, const long long* d_
         lds = want < lds_limit ? want : lds_limit;
     }
     if (lds < base) lds = base;
-    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, d_inc, maxw, (int)lds, budget_ticks, flags);
+    if (budget_ticks > 0) {
+        if (!launch_t0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_clock_reset, dim3(1), dim3(1), 0, s, launch_t0);
+    }
+    hipLaunchKernelGGL(k_run, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, d_inc, maxw, (int)lds, budget_ticks, flags,
+                       launch_t0);
     return hipGetLastError();
 }
 int engine_max_waves(int max_n, int maxw, size_t lds_limit) {

@@ -201,13 +201,45 @@ def read_checks(stdout):
     return out
 
 
-def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1):
+def mirrored(sysd, box=64.0, grid=2.0 ** -20):
+    """The planted-tie start: the first chain of each species and a copy of
+    it translated by half a box along x, with the copy's velocities equal to
+    the original's, in a box of a power-of-two length (Angstrom to box units
+    is then exact) with positions on a 2^-20 grid, so that every collision
+    time of a copy's pair equals the original pair's bit for bit."""
+    sv = np.array(sysd["sv"], np.float64).reshape(-1, 6).copy()
+    B0 = float(sysd["boxl"])
+    n1, n2 = 4 * L1, 4 * L2
+    starts = [(0, n1), (2 * n1, n2)]  # chain 1 and chain 3 (first of species 2)
+    for s0, n in starts:
+        # the chain made contiguous around its first bead (minimum image in
+        # the generator's box), then wrapped into the new box
+        rel = sv[s0:s0 + n, :3] - sv[s0, :3]
+        rel = rel - np.round(rel)
+        x = (sv[s0, :3] + rel) * B0 / box
+        x = x - np.round(x)
+        x = np.round(x / grid) * grid
+        sv[s0:s0 + n, :3] = x
+        cp = x.copy()
+        cp[:, 0] = cp[:, 0] + 0.5
+        cp = cp - np.round(cp)
+        sv[s0 + n:s0 + 2 * n, :3] = cp
+        sv[s0 + n:s0 + 2 * n, 3:] = sv[s0:s0 + n, 3:]
+    out = dict(sysd)
+    out["sv"] = sv.ravel()
+    out["boxl"] = box
+    return out
+
+
+def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1, mirror=False):
     import pdmd
     rng = np.random.default_rng(seed)
     seq1 = [int(x) for x in rng.integers(9, 29, L1)]
     seq2 = [int(x) for x in rng.integers(9, 29, L2)]
     seq = seq1 * NCH1 + seq2 * NCH2
     sysd = pdmd.generate([L1] * NCH1 + [L2] * NCH2, density=density, tstar=tstar, seed=seed, sequence=seq)
+    if mirror:
+        sysd = mirrored(sysd)
     work = os.path.join(OUT, "work_" + name)
     if os.path.isdir(work):
         shutil.rmtree(work)
@@ -242,6 +274,10 @@ def main():
     # branch of the worker's resolution (coltypes 7, 10, 12 -> 4/5/6/9/13/
     # 14/15/16 -> 20-27), shoulders added and removed, end-bead capture draws
     run_case("ref_2species_hb", exe, seed=21, tstar=0.18, density=0.30, nsteps=1000000)
+    # planted ties: two translated copies of each species' first chain, so
+    # that calendar entries of copy and original share their times bit for
+    # bit and the reference's LIFO bucket lists decide which comes first
+    run_case("ref_2species_tie", exe, seed=31, tstar=0.20, density=0.015, nsteps=200000, mirror=True)
 
 
 if __name__ == "__main__":

@@ -4,8 +4,12 @@ The fixtures in tests/golden/ come from oracle/_ref/ref_harness (the
 reference's unmodified Fortran compiled by oracle/ref/Makefile, driven by
 tests/golden/make_golden.py) on a two-species system (2 chains x 8 residues
 and 2 chains x 6 residues): rho* = 0.01, T* = 0.20, built plain, with
--Dno_hbs and with -Dno_h, and rho* = 0.30, T* = 0.18 plain, where N-C
-hydrogen bonds form and break.  They hold the scaled parameter tables
+-Dno_hbs and with -Dno_h, rho* = 0.30, T* = 0.18 plain, where N-C hydrogen
+bonds form and break, and a planted-tie start (each species' first chain and
+a copy translated by half a box, positions on a 2^-20 grid in a 64 A box) in
+which copy and original share collision times bit for bit, so that the
+reference's LIFO bucket lists (add_tbin.f:28-32, main.F90:521-545) order the
+tied entries.  They hold the scaled parameter tables
 (inputinfo.f + scale_down.f), ev_code (make_code.f), the neighbour lists
 (nbor_setup.f, cell_link.f, nbor.f), the calendar after events.f, every pair
 prediction (core/bond/sqwel/sqshlder/nc_sqwel.f), the energy and shape
@@ -30,7 +34,7 @@ from oracle import Oracle
 HERE = os.path.dirname(os.path.abspath(__file__))
 # (fixture, build flags of the reference harness that wrote it)
 CASES = [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}), ("ref_2species_noh", {"no_h": True}),
-         ("ref_2species_hb", {})]
+         ("ref_2species_hb", {}), ("ref_2species_tie", {})]
 # coltypes of the worker's N-C resolution (main.F90:1487-1937, eventdyn.f)
 HB_TYPES = (14, 15, 16, 23, 24, 25, 26)
 LOG_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("t", "<f8")])
@@ -188,6 +192,20 @@ def test_serial_loop(param_dir):
         assert s.rng_ctr[1] == fx["loop_draws"], name
         _obs_close(orc.observables(), fx["obs_f"][1], fx["obs_i"][1])
         _checks_match(orc.check, fx, name)
+
+
+def test_tie_fixture_holds_ties_in_lifo_order():
+    """The planted-tie fixture's loop commits entries of equal time back to
+    back, the later-added one first: after events.f's adds in index order
+    (events.f:111-122) the copy's bead (index + 32) precedes the original."""
+    fx = _load("ref_2species_tie")
+    N = fx["identity"].size
+    tim = fx["tim0"][:N]
+    assert (tim[:32] == tim[32:64]).all()
+    t, i = fx["loop_t"], fx["loop_i"]
+    eq = np.nonzero((t[1:] == t[:-1]) & (i[1:] <= N) & (i[:-1] <= N))[0]
+    assert eq.size >= 8
+    assert sum(int(i[k]) == int(i[k + 1]) + 32 for k in eq) >= 4
 
 
 def test_hbond_fixture_exercises_the_resolution():
