@@ -32,6 +32,12 @@ typedef struct pdmd_ctx pdmd_ctx;
  * ghost collisions (-Dcanon). */
 #define PDMD_NO_HBS 1 /* no N-C hydrogen-bond wells: the reference's code 15 pairs stay hard cores */
 #define PDMD_NO_H 2   /* build flag -Dno_h (inputinfo.f:270-280): no code-16 wells between chains; same-chain ones stay */
+/* build flag -Dchaptype=2: the chains after the first species block (the
+ * reference's nop2 block) carry no bonds, local cores or intra-chain wells
+ * (make_code.f:349-484, inputinfo.f:120-160) and no side-chain wells with
+ * each other (make_code.f:519-547); checkover.f skips bond windows (:67).
+ * Their beads start apart, as unbonded hard cores must. */
+#define PDMD_CHAPTYPE2 4
 
 typedef struct {
     int32_t nchains;
@@ -44,7 +50,8 @@ typedef struct {
     uint64_t seed;
     int32_t canon;
     int32_t list_capacity; /* per-bead up/down list slots, at most 64; 0 = default (64) */
-    int32_t flags;         /* PDMD_NO_HBS (-Dno_hbs, make_code.f:114-170,192-208,348-365) | PDMD_NO_H (-Dno_h) */
+    int32_t flags;         /* PDMD_NO_HBS (-Dno_hbs, make_code.f:114-170,192-208,348-365) | PDMD_NO_H (-Dno_h)
+                              | PDMD_CHAPTYPE2 (-Dchaptype=2) */
     int32_t pad;
     const double* sv;
     const int32_t* bptnr;

@@ -90,7 +90,8 @@ def _p(a, t):
 class Oracle:
     """One system in the CPU oracle."""
 
-    def __init__(self, system, seed, param_dir, canon=True, bptnr=None, species=None, no_hbs=False, no_h=False):
+    def __init__(self, system, seed, param_dir, canon=True, bptnr=None, species=None, no_hbs=False, no_h=False,
+                 chaptype2=False):
         self.cl = np.ascontiguousarray(system["chain_len"], np.int32)
         self.rt = np.ascontiguousarray(system["restype"], np.int32)
         self.hp = np.ascontiguousarray(system["hp"], np.int32)
@@ -102,7 +103,7 @@ class Oracle:
         self.h = lib().orc_create_ex(param_dir.encode(), len(self.cl), _p(self.cl, ctypes.c_int32),
                                      _p(self.sp, ctypes.c_int32), _p(self.rt, ctypes.c_int32),
                                      _p(self.hp, ctypes.c_int32), float(system["boxl"]), float(system["tstar"]),
-                                     int(seed), 1 if canon else 0, (1 if no_hbs else 0) | (2 if no_h else 0), _p(self.sv, ctypes.c_double),
+                                     int(seed), 1 if canon else 0, (1 if no_hbs else 0) | (2 if no_h else 0) | (4 if chaptype2 else 0), _p(self.sv, ctypes.c_double),
                                      _p(self.bp, ctypes.c_int32), err, 256)
         if not self.h:
             raise RuntimeError("orc_create failed: " + err.value.decode())

@@ -107,6 +107,8 @@ struct orc_sys {
     int canon;
     int no_hbs; /* -Dno_hbs: no N-C hydrogen-bond wells (make_code.f:114-170,192-208,348-365) */
     int no_h;   /* -Dno_h: hp zeroed where make_code.f:80-104 reads it (inputinfo.f:270-280) */
+    int chaptype2; /* -Dchaptype=2 for the chains after the first species block */
+    int nfirst;    /* chains of the first species block (the reference's nop1 block) */
     int64_t coll, numghosts, nupdates, nforced, nevents[31];
     uint64_t seed, rng_ctr[2];
     double old_tfalse;
@@ -386,8 +388,12 @@ static void make_code(orc_sys* S) {
                     EV(S, l - 2 * S->cl[l], m) = 1;
             }
     }
-    /* per chain block (:167-329 species 1, :331-484 species 2) */
+    /* per chain block (:167-329 species 1, :331-484 species 2; with
+     * -Dchaptype=2 the second block's side chains have no firstside2.data
+     * read (inputinfo.f:120-132) and its H-bond and bond codes are skipped
+     * (:349,366), so it sets nothing) */
     for (int c = 0; c < S->nchains; ++c) {
+        if (S->chaptype2 && c >= S->nfirst) continue;
         int lll = S->chain_start[c], L = S->chain_len[c];
 #define FS(r) (3 * L + (r)) /* fside(r): firstside data all 1 */
         int ncount = 2 + N_B_HYDRO;
@@ -428,6 +434,20 @@ static void make_code(orc_sys* S) {
         for (int k = lll + 2 * L + 1; k <= lll + 3 * L - 2; ++k)
             EV(S, k, lll + FS(k - lll - 2 * L + 2)) = 26;
 #undef FS
+    }
+    if (S->chaptype2 && S->nfirst < S->nchains) {
+        /* :489-517: side-chain wells between chains again, over all beads */
+        for (int m = 1; m <= N - 1; ++m)
+            for (int l = m + 1; l <= N; ++l)
+                if (!S->no_h && id[m] > 8 && id[l] > 8 && S->hp_bead[m] == 1 && S->hp_bead[l] == 1 &&
+                    S->chnnum[l] != S->chnnum[m])
+                    EV(S, m, l) = 16;
+        /* :519-547: none between two chains of the second block */
+        for (int m = S->chain_start[S->nfirst] + 1; m <= N - 1; ++m)
+            for (int l = m + 1; l <= N; ++l)
+                if (!S->no_h && id[m] > 8 && id[l] > 8 && S->hp_bead[m] == 1 && S->hp_bead[l] == 1 &&
+                    S->chnnum[l] != S->chnnum[m])
+                    EV(S, m, l) = 1;
     }
     /* symmetrise from the upper triangle (:558-563) */
     for (int i = 1; i <= N - 1; ++i)
@@ -1763,7 +1783,7 @@ int32_t orc_checkover(const orc_sys* S) {
                 double r2 = g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
                 r2 = r2 * 1.0000000001;
                 if (r2 <= sigsq) bad++;
-            } else if (evcode >= 4 && evcode <= 12) {
+            } else if (evcode >= 4 && evcode <= 12 && !S->chaptype2) { /* checkover.f:67: chaptype 1 only */
                 double blmin, blmax;
                 bond_window(S, i, evcode, &blmin, &blmax);
                 blmin = blmin * blmin;
@@ -1807,7 +1827,7 @@ void orc_check(orc_sys* S, const double* sv, int64_t out[9]) {
                     out[0]++;
                     bad = 1;
                 }
-            } else if (evcode >= 4 && evcode <= 12) {
+            } else if (evcode >= 4 && evcode <= 12 && !S->chaptype2) { /* checkover.f:67: chaptype 1 only */
                 double blmin, blmax;
                 bond_window(S, i, evcode, &blmin, &blmax);
                 blmin = blmin * blmin;
@@ -1894,10 +1914,13 @@ orc_sys* orc_create_ex(const char* param_dir, int nchains, const int32_t* chain_
         N += 4 * chain_len[c];
         nres += chain_len[c];
     }
+    S->nfirst = 0;
+    while (S->nfirst < nchains && S->species[S->nfirst] == S->species[0]) S->nfirst++;
     S->N = N;
     S->canon = canon;
     S->no_hbs = (flags & ORC_NO_HBS) != 0;
     S->no_h = (flags & ORC_NO_H) != 0;
+    S->chaptype2 = (flags & ORC_CHAPTYPE2) != 0;
     S->seed = seed;
     size_t n1 = (size_t)N + 4;
     S->chnnum = (int*)calloc(n1, sizeof(int));

@@ -62,6 +62,12 @@ orc_sys* orc_create(const char* param_dir, int nchains, const int32_t* chain_len
  * which removes the code-16 wells between chains (make_code.f:80-104); the
  * same-chain ones read hp1/hp2 and stay. */
 #define ORC_NO_H 2
+/* ORC_CHAPTYPE2 restates -Dchaptype=2: the chains after the first species
+ * block (the reference's nop2 block) get no intra-chain codes (no bonds,
+ * local cores, H-bond or side-chain wells, make_code.f:349-484) and no
+ * side-chain wells with each other (make_code.f:519-547); checkover.f skips
+ * its bond-window checks (checkover.f:67). */
+#define ORC_CHAPTYPE2 4
 orc_sys* orc_create_ex(const char* param_dir, int nchains, const int32_t* chain_len,
                        const int32_t* species, const int32_t* restype, const int32_t* hp,
                        double boxl_angstrom, double tstar, uint64_t seed, int canon, int flags,

@@ -177,7 +177,7 @@ int validate_desc(const pdmd_system_desc* d) {
     for (int c = 0; c < d->nchains; ++c)
         if (d->chain_len[c] < 4) return fail("pdmd_system_desc: chains need at least 4 residues");
     if (!(d->boxl > 0) || !(d->tstar > 0)) return fail("pdmd_system_desc: boxl and tstar must be positive");
-    if (d->flags & ~(PDMD_NO_HBS | PDMD_NO_H)) return fail("pdmd_system_desc: unknown flags");
+    if (d->flags & ~(PDMD_NO_HBS | PDMD_NO_H | PDMD_CHAPTYPE2)) return fail("pdmd_system_desc: unknown flags");
     return 0;
 }
 
@@ -209,13 +209,14 @@ int prepare_system(const Params& raw, const pdmd_system_desc* d, Prep& out) {
     H.topo.build(clen, spec, rt, hp);
     H.topo.no_hbs = (d->flags & PDMD_NO_HBS) != 0;
     H.topo.no_h = (d->flags & PDMD_NO_H) != 0;
+    H.topo.chaptype2 = (d->flags & PDMD_CHAPTYPE2) != 0;
     const Topology& t = H.topo;
     const int N = t.N;
     H.N = N;
     H.cap = d->list_capacity > 0 ? d->list_capacity : 64;
     if (H.cap > 64) return fail("pdmd_system_add: list_capacity above 64 (one wavefront lane per list slot)");
     if (N > 65533) return fail("pdmd_system_add: at most 65533 beads per system");
-    if (nc > 4095) return fail("pdmd_system_add: at most 4095 chains per system");
+    if (nc > 2047) return fail("pdmd_system_add: at most 2047 chains per system");
     for (int c = 0; c < nc; ++c)
         if (clen[c] > 128) return fail("pdmd_system_add: at most 128 residues per chain");
     H.canon = d->canon ? 1 : 0;
@@ -366,7 +367,8 @@ int prepare_system(const Params& raw, const pdmd_system_desc* d, Prep& out) {
         const int c = t.chain[k], L = t.chain_len[c], kind = (t.loc[k] - 1) / L, r = (t.loc[k] - 1) % L + 1;
         const int q = t.res[k];
         const int pro = rt[q] == 17 ? 1 : 0;
-        bead[k].topo = make_topo(c, kind, r, L, hp[q] ? 1 : 0, pro, t.no_hbs ? 1 : 0, t.no_h ? 1 : 0);
+        bead[k].topo = make_topo(c, kind, r, L, hp[q] ? 1 : 0, pro, t.no_hbs ? 1 : 0, t.no_h ? 1 : 0,
+                                 t.free_bead(k) ? 1 : 0);
         bead[k].dyn = make_dyn(ident[k], bptnr[k]);
         if (kind <= 2) {
             const int iii = rt[q] - 8;
@@ -554,6 +556,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.nres = nres;
     D.cap = H.cap;
     D.canon = H.canon;
+    D.chaptype2 = H.topo.chaptype2 ? 1 : 0;
     D.M = M;
     D.Mq = Mq;
     D.ncoarse = Mq * Mq * Mq;
@@ -1095,6 +1098,7 @@ int pdmd_pair_codes(const pdmd_system_desc* d, int32_t n, const int32_t* a, cons
     t.build(clen, spec, rt, hp);
     t.no_hbs = (d->flags & PDMD_NO_HBS) != 0;
     t.no_h = (d->flags & PDMD_NO_H) != 0;
+    t.chaptype2 = (d->flags & PDMD_CHAPTYPE2) != 0;
     for (int k = 0; k < n; ++k) {
         if (a[k] < 1 || a[k] > t.N || b[k] < 1 || b[k] > t.N) return fail("pdmd_pair_codes: bead index out of range");
         code[k] = a[k] == b[k] ? 1 : static_code(t, a[k] - 1, b[k] - 1);

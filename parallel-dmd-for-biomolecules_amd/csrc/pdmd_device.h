@@ -18,8 +18,9 @@
 
 namespace pdmd {
 
-// topo word: chain (12 bits: at most 4095 chains of >= 16 beads fit in
-// 65533) | system built with -Dno_h (1) | kind (2: 0 Calpha, 1 N, 2 C, 3 R)
+// topo word: chain (11 bits: at most 2047 chains) | bead of a chain built
+// as -Dchaptype=2 builds the second species (1: no bonds or intra-chain
+// wells) | system built with -Dno_h (1) | kind (2: 0 Calpha, 1 N, 2 C, 3 R)
 // | residue-1 (7) | chain length-1 (7) | hp (1) | proline residue (1) |
 // system built with -Dno_hbs (1, the sign bit)
 // dyn word:  identity (5 bits) | (H-bond partner + 1) << 5
@@ -34,7 +35,9 @@ struct __attribute__((aligned(16))) Leaf {
     int meta;  // (bin + 1) in the low 16 bits, coltype in the high 16 (signed)
 };
 
-__host__ __device__ inline int topo_chain(int t) { return t & 0xFFF; }
+__host__ __device__ inline int topo_chain(int t) { return t & 0x7FF; }
+// -Dchaptype=2: the bead's chain is of the unbonded second species
+__host__ __device__ inline int topo_free(int t) { return (t >> 11) & 1; }
 // -Dno_h: side chains of different chains never form code-16 wells
 __host__ __device__ inline int topo_noh(int t) { return (t >> 12) & 1; }
 __host__ __device__ inline int topo_kind(int t) { return (t >> 13) & 3; }
@@ -43,9 +46,10 @@ __host__ __device__ inline int topo_len(int t) { return ((t >> 22) & 127) + 1; }
 __host__ __device__ inline int topo_hp(int t) { return (t >> 29) & 1; }
 __host__ __device__ inline int topo_pro(int t) { return (t >> 30) & 1; }
 __host__ __device__ inline int topo_nohbs(int t) { return t < 0 ? 1 : 0; }
-__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro, int nohbs, int noh) {
-    return (int)((unsigned)((chain & 0xFFF) | (noh << 12) | (kind << 13) | ((r - 1) << 15) | ((L - 1) << 22) | (hp << 29) |
-                            (pro << 30)) |
+__host__ __device__ inline int make_topo(int chain, int kind, int r, int L, int hp, int pro, int nohbs, int noh,
+                                         int free_ = 0) {
+    return (int)((unsigned)((chain & 0x7FF) | (free_ << 11) | (noh << 12) | (kind << 13) | ((r - 1) << 15) |
+                            ((L - 1) << 22) | (hp << 29) | (pro << 30)) |
                  ((unsigned)nohbs << 31));
 }
 __host__ __device__ inline int dyn_ident(int d) { return d & 31; }
@@ -91,6 +95,7 @@ struct LogRec {
 struct SysDev {
     int N, nres, cap, canon;
     int M, Mq, ncoarse, log_cap;  // fine cells per axis (interior), coarse cells per axis
+    int chaptype2, pad;           // built as -Dchaptype=2 (checkover.f:67 then skips bond windows)
     double width, half, setemp, avegtime, hdelr;
     unsigned long long seed;
     const DevTables* tab;

@@ -199,6 +199,9 @@ __device__ double dlog(double x) {
 __device__ inline int static_t0(int ta, int tb) {
     int ka = topo_kind(ta), ra = topo_res(ta), kb = topo_kind(tb), rb = topo_res(tb);
     if (topo_chain(ta) == topo_chain(tb)) {
+        // -Dchaptype=2 builds no code inside a second-species chain
+        // (make_code.f:349,366; its side-chain loop finds no firstside2 entry)
+        if (topo_free(ta)) return 1;
         if (ka > kb || (ka == kb && ra > rb)) {
             int t = ka;
             ka = kb;
@@ -228,7 +231,8 @@ __device__ inline int static_t0(int ta, int tb) {
     }
     // -Dno_h zeroes hp only where make_code.f:80-104 reads it (different
     // chains); the same-chain loops read hp1/hp2 (make_code.f:174-186)
-    if (ka == 3 && kb == 3) return (topo_hp(ta) && topo_hp(tb) && !topo_noh(ta)) ? 16 : 1;
+    // and none between two second-species chains (make_code.f:519-547)
+    if (ka == 3 && kb == 3) return (topo_hp(ta) && topo_hp(tb) && !topo_noh(ta) && !(topo_free(ta) && topo_free(tb))) ? 16 : 1;
     if (ka == 1 && kb == 2) return (topo_pro(ta) && topo_chain(ta) < topo_chain(tb)) ? 1 : 15;
     if (ka == 2 && kb == 1) return (topo_pro(tb) && topo_chain(tb) < topo_chain(ta)) ? 1 : 15;
     return 1;
@@ -3732,7 +3736,7 @@ __global__ void k_check(const SysDev* sys, int id, const Rec* alt, unsigned long
                 ++over;
                 first = min(first, (unsigned long long)i * (unsigned long long)N + (unsigned long long)j);
             }
-        } else if (code >= 4 && code <= 12) {
+        } else if (code >= 4 && code <= 12 && !S.chaptype2) {  // checkover.f:67: chaptype 1 only
             double blmin, blmax;
             bond_window(T, code, S.bw[i], blmin, blmax);
             blmin = blmin * blmin;
@@ -4047,7 +4051,7 @@ hipError_t engine_observables(const SysDev* d_sys, int id, const int* cstart, co
 // observables without device allocation: the chain tables and the results
 // are staged in one fixed device array per GPU
 // ------------------------------------------------------------------------
-constexpr int kScratchChains = 4096;  // a system holds at most 4095 chains
+constexpr int kScratchChains = 4096;  // a system holds at most 2047 chains
 constexpr size_t kScratchClen = sizeof(int) * kScratchChains;
 constexpr size_t kScratchOutd = 2 * kScratchClen;
 constexpr size_t kScratchOuti = kScratchOutd + 64;

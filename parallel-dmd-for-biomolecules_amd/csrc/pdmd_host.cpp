@@ -203,6 +203,8 @@ void Topology::build(const std::vector<int>& clen, const std::vector<int>& spec,
         N += 4 * clen[c];
         nres += clen[c];
     }
+    nfirst = 0;
+    while (nfirst < nc && spec[nfirst] == spec[0]) ++nfirst;
     chain.assign(N, 0);
     loc.assign(N, 0);
     res.assign(N, 0);
@@ -236,6 +238,9 @@ int static_code(const Topology& t, int a, int b) {
     kind_of(t, a, ka, ra);
     kind_of(t, b, kb, rb);
     if (t.chain[a] == t.chain[b]) {
+        // -Dchaptype=2: nothing inside a second-species chain (make_code.f:
+        // 349,366; no firstside2.data is read, inputinfo.f:120-132)
+        if (t.free_bead(a)) return 1;
         if (ka > kb || (ka == kb && ra > rb)) {
             std::swap(ka, kb);
             std::swap(ra, rb);
@@ -263,7 +268,9 @@ int static_code(const Topology& t, int a, int b) {
     // different chains
     // -Dno_h zeroes hp for make_code.f:80-104 only (inputinfo.f:270-280);
     // the same-chain 16s above come from hp1/hp2, which it leaves alone
-    if (ka == 3 && kb == 3) return (!t.no_h && t.hp[t.res[a]] == 1 && t.hp[t.res[b]] == 1) ? 16 : 1;
+    // -Dchaptype=2: none between two second-species chains (make_code.f:519-547)
+    if (ka == 3 && kb == 3)
+        return (!t.no_h && t.hp[t.res[a]] == 1 && t.hp[t.res[b]] == 1 && !(t.free_bead(a) && t.free_bead(b))) ? 16 : 1;
     if ((ka == 1 && kb == 2) || (ka == 2 && kb == 1)) {
         if (t.no_hbs) return 1;  // make_code.f:114-170 skipped under -Dno_hbs
         // proline NH rule (make_code.f:139-161): kept only where the Pro side

@@ -45,6 +45,9 @@ struct Topology {
     std::vector<int> chain, loc, res, ident0;            // per bead (0-based chain/res; loc 1-based)
     bool no_hbs = false;                                 // -Dno_hbs: no code-15 N-C wells
     bool no_h = false;                                   // -Dno_h: no inter-chain code-16 wells
+    bool chaptype2 = false;                              // -Dchaptype=2: chains after the first species block unbonded
+    int nfirst = 0;                                      // chains of the first species block
+    bool free_bead(int b) const { return chaptype2 && chain[b] >= nfirst; }
     void build(const std::vector<int>& chain_len, const std::vector<int>& species,
                const std::vector<int>& restype, const std::vector<int>& hp);
 };

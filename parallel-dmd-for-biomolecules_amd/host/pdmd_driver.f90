@@ -25,6 +25,7 @@
 !     waves = 1             ! wavefronts per system (1 to 8; same results)
 !     no_hbs = 0            ! 1: build flag -Dno_hbs (no N-C H-bond wells)
 !     no_h = 0              ! 1: build flag -Dno_h (no side-chain wells between chains)
+!     chaptype = 1          ! 2: build flag -Dchaptype=2 (second species unbonded)
 !     debugging = 0         ! 1: -Ddebugging checks at every output event
 !   /
 !
@@ -39,11 +40,11 @@ module pdmd_driver_m
   character(len=16) :: mode = 'run', start = 'helix'
   character(len=256) :: param_dir = 'parameters'
   integer :: nchains1 = 8, chnln1 = 16, nchains2 = 0, chnln2 = 0, canon = 1, list_capacity = 0, waves = 1
-  integer :: no_hbs = 0, no_h = 0, debugging = 0
+  integer :: no_hbs = 0, no_h = 0, debugging = 0, chaptype = 1
   integer(c_int64_t) :: seed = 1, gen_seed = 1
   real(c_double) :: boxl = 0.0d0, density = 0.01d0
   namelist /pdmd_run/ mode, nchains1, chnln1, nchains2, chnln2, start, boxl, canon, seed, param_dir, density, &
-                      gen_seed, list_capacity, waves, no_hbs, no_h, debugging
+                      gen_seed, list_capacity, waves, no_hbs, no_h, debugging, chaptype
 
   real(c_double) :: tstar, setemp, sigma1
   integer(c_int64_t) :: ncoll
@@ -307,7 +308,8 @@ contains
     desc%seed = seed
     desc%canon = int(canon, c_int32_t)
     desc%list_capacity = int(list_capacity, c_int32_t)
-    desc%flags = merge(1_c_int32_t, 0_c_int32_t, no_hbs /= 0) + merge(2_c_int32_t, 0_c_int32_t, no_h /= 0)
+    desc%flags = merge(1_c_int32_t, 0_c_int32_t, no_hbs /= 0) + merge(2_c_int32_t, 0_c_int32_t, no_h /= 0) &
+                 + merge(4_c_int32_t, 0_c_int32_t, chaptype == 2)
     desc%sv = c_loc(sv)
     desc%bptnr = c_loc(bptnr)
     pd = trim(param_dir) // c_null_char
@@ -401,7 +403,8 @@ contains
       desc%seed = seed
       desc%canon = int(canon, c_int32_t)
       desc%list_capacity = 0
-      desc%flags = merge(1_c_int32_t, 0_c_int32_t, no_hbs /= 0) + merge(2_c_int32_t, 0_c_int32_t, no_h /= 0)
+      desc%flags = merge(1_c_int32_t, 0_c_int32_t, no_hbs /= 0) + merge(2_c_int32_t, 0_c_int32_t, no_h /= 0) &
+                 + merge(4_c_int32_t, 0_c_int32_t, chaptype == 2)
       desc%sv = c_loc(sv)
       desc%bptnr = c_null_ptr
       ! the first two chains: same-chain and inter-chain pairs

@@ -22,6 +22,7 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 
 NO_HBS = 1  # pdmd_system_desc.flags: build flag -Dno_hbs
 NO_H = 2  # pdmd_system_desc.flags: build flag -Dno_h (every side chain polar)
+CHAPTYPE2 = 4  # pdmd_system_desc.flags: build flag -Dchaptype=2 (second species unbonded)
 
 
 class SystemDesc(ctypes.Structure):
@@ -229,9 +230,9 @@ def generate(chain_len, density=0.01, tstar=0.20, seed=1, param_dir=PARAM_DIR, s
             "density": density}
 
 
-def pair_codes(system, a, b, species=None, no_hbs=False, no_h=False):
+def pair_codes(system, a, b, species=None, no_hbs=False, no_h=False, chaptype2=False):
     """Static pair codes ev_code(a, b) (1-based beads) of a system; host only."""
-    d, keep = Engine.make_desc(system, 0, species=species, no_hbs=no_hbs, no_h=no_h)
+    d, keep = Engine.make_desc(system, 0, species=species, no_hbs=no_hbs, no_h=no_h, chaptype2=chaptype2)
     a = np.ascontiguousarray(a, np.int32)
     b = np.ascontiguousarray(b, np.int32)
     out = np.zeros(a.size, np.int32)
@@ -296,7 +297,8 @@ class Engine:
             pass
 
     @staticmethod
-    def make_desc(system, seed, canon=True, bptnr=None, species=None, list_capacity=0, no_hbs=False, no_h=False):
+    def make_desc(system, seed, canon=True, bptnr=None, species=None, list_capacity=0, no_hbs=False, no_h=False,
+                  chaptype2=False):
         keep = dict(cl=np.ascontiguousarray(system["chain_len"], np.int32),
                     rt=np.ascontiguousarray(system["restype"], np.int32),
                     hp=np.ascontiguousarray(system["hp"], np.int32),
@@ -306,12 +308,13 @@ class Engine:
         d = SystemDesc(len(keep["cl"]), _p(keep["cl"], ctypes.c_int32), _p(keep["sp"], ctypes.c_int32),
                        _p(keep["rt"], ctypes.c_int32), _p(keep["hp"], ctypes.c_int32), float(system["boxl"]),
                        float(system["tstar"]), int(seed), 1 if canon else 0, int(list_capacity),
-                       (NO_HBS if no_hbs else 0) | (NO_H if no_h else 0), 0, _p(keep["sv"], ctypes.c_double), _p(keep["bp"], ctypes.c_int32))
+                       (NO_HBS if no_hbs else 0) | (NO_H if no_h else 0) | (CHAPTYPE2 if chaptype2 else 0), 0,
+                       _p(keep["sv"], ctypes.c_double), _p(keep["bp"], ctypes.c_int32))
         return d, keep
 
     def add_system(self, system, seed, canon=True, bptnr=None, species=None, list_capacity=0, no_hbs=False,
-                   no_h=False):
-        d, keep = self.make_desc(system, seed, canon, bptnr, species, list_capacity, no_hbs, no_h)
+                   no_h=False, chaptype2=False):
+        d, keep = self.make_desc(system, seed, canon, bptnr, species, list_capacity, no_hbs, no_h, chaptype2)
         sid = ctypes.c_int32(0)
         _check(lib().pdmd_system_add(self._ctx, ctypes.byref(d), ctypes.byref(sid)))
         self._keep[sid.value] = keep
@@ -403,8 +406,9 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, system, ncoll, seed, canon=True, species=None, bptnr=None, no_hbs=False, no_h=False):
-        d, keep = Engine.make_desc(system, seed, canon, bptnr, species, no_hbs=no_hbs, no_h=no_h)
+    def submit(self, system, ncoll, seed, canon=True, species=None, bptnr=None, no_hbs=False, no_h=False,
+               chaptype2=False):
+        d, keep = Engine.make_desc(system, seed, canon, bptnr, species, no_hbs=no_hbs, no_h=no_h, chaptype2=chaptype2)
         rid = ctypes.c_int64(0)
         _check(lib().pdmd_server_submit(self._h, ctypes.byref(d), int(ncoll), ctypes.byref(rid)))
         return rid.value

@@ -231,7 +231,28 @@ def mirrored(sysd, box=64.0, grid=2.0 ** -20):
     return out
 
 
-def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1, mirror=False):
+def scattered(sysd, seed, gap=4.5):
+    """A start for -Dchaptype=2, where the second species' beads are not
+    bonded and every pair of them is a hard core: its beads moved to random
+    places at least `gap` Angstrom from every other bead (velocities kept)."""
+    rng = np.random.default_rng(seed)
+    sv = np.array(sysd["sv"], np.float64).reshape(-1, 6).copy()
+    B = float(sysd["boxl"])
+    n1 = 4 * L1 * NCH1
+    for k in range(n1, sv.shape[0]):
+        while True:
+            x = rng.uniform(-0.5, 0.5, 3)
+            d = sv[:k, :3] - x
+            d = d - np.round(d)
+            if np.sqrt((d * d).sum(1)).min() * B >= gap:
+                sv[k, :3] = x
+                break
+    out = dict(sysd)
+    out["sv"] = sv.ravel()
+    return out
+
+
+def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1, mirror=False, scatter=False):
     import pdmd
     rng = np.random.default_rng(seed)
     seq1 = [int(x) for x in rng.integers(9, 29, L1)]
@@ -240,6 +261,8 @@ def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=
     sysd = pdmd.generate([L1] * NCH1 + [L2] * NCH2, density=density, tstar=tstar, seed=seed, sequence=seq)
     if mirror:
         sysd = mirrored(sysd)
+    if scatter:
+        sysd = scattered(sysd, seed)
     work = os.path.join(OUT, "work_" + name)
     if os.path.isdir(work):
         shutil.rmtree(work)
@@ -265,7 +288,8 @@ def main():
     exe = os.path.join(OUT, "ref_harness")
     exe_n = os.path.join(OUT, "ref_harness_nohbs")
     exe_h = os.path.join(OUT, "ref_harness_noh")
-    if not all(os.path.exists(e) for e in (exe, exe_n, exe_h)):
+    exe_c2 = os.path.join(OUT, "ref_harness_ch2")
+    if not all(os.path.exists(e) for e in (exe, exe_n, exe_h, exe_c2)):
         raise SystemExit("build the checker first: make -C oracle/ref")
     run_case("ref_2species", exe, seed=4242)
     run_case("ref_2species_nohbs", exe_n, seed=4243)
@@ -278,6 +302,8 @@ def main():
     # that calendar entries of copy and original share their times bit for
     # bit and the reference's LIFO bucket lists decide which comes first
     run_case("ref_2species_tie", exe, seed=31, tstar=0.20, density=0.015, nsteps=200000, mirror=True)
+    # -Dchaptype=2: the second species (2 chains x 6 residues) unbonded
+    run_case("ref_2species_ch2", exe_c2, seed=4245, tstar=0.20, density=0.01, nsteps=200000, scatter=True)
 
 
 if __name__ == "__main__":

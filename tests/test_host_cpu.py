@@ -37,7 +37,8 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("name,flags", [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}),
-                                        ("ref_2species_noh", {"no_h": True})])
+                                        ("ref_2species_noh", {"no_h": True}),
+                                        ("ref_2species_ch2", {"chaptype2": True})])
 def test_pair_codes_match_reference_make_code(name, flags):
     fx = dict(np.load(os.path.join(ROOT, "tests", "golden", name + ".npz")))
     sysd = dict(chain_len=fx["chain_len"], restype=fx["restype"], hp=fx["hp"], sv=fx["sv0"], boxl=float(fx["boxl"]),
@@ -54,10 +55,26 @@ def test_pair_codes_match_reference_make_code(name, flags):
     assert bad.size == 0, f"first differing pair {bad[0] + 1}: {got[tuple(bad[0])]} vs {ref[tuple(bad[0])]}"
 
 
-@pytest.mark.parametrize("seed,chains", [(1, [16] * 8), (2, [8, 12, 24, 32, 5]), (3, [4] * 20)])
-def test_pair_codes_match_oracle(param_dir, seed, chains):
+@pytest.mark.parametrize("seed,chains,ch2", [(1, [16] * 8, False), (2, [8, 12, 24, 32, 5], False), (3, [4] * 20, False),
+                                            (4, [8, 12, 24, 32, 5], True)])
+def test_pair_codes_match_oracle(param_dir, seed, chains, ch2):
     sysd = pdmd.generate(chains, density=0.01, tstar=0.2, seed=seed)
-    orc = Oracle(sysd, seed=1, param_dir=param_dir)
+    # with -Dchaptype=2 the first two chains form the first species block;
+    # the unbonded beads of the others start apart (4.5 A from every bead)
+    species = np.array([0, 0] + list(range(1, len(chains) - 1)), np.int32) if ch2 else None
+    if ch2:
+        rng = np.random.default_rng(seed)
+        sv = np.array(sysd["sv"]).reshape(-1, 6).copy()
+        for k in range(4 * (chains[0] + chains[1]), sv.shape[0]):
+            while True:
+                x = rng.uniform(-0.5, 0.5, 3)
+                d = sv[:k, :3] - x
+                d -= np.round(d)
+                if np.sqrt((d * d).sum(1)).min() * sysd["boxl"] >= 4.5:
+                    sv[k, :3] = x
+                    break
+        sysd = dict(sysd, sv=sv.ravel())
+    orc = Oracle(sysd, seed=1, param_dir=param_dir, species=species, chaptype2=ch2)
     N = orc.N
     rng = np.random.default_rng(seed)
     a = rng.integers(1, N + 1, 20000)
@@ -72,7 +89,7 @@ def test_pair_codes_match_oracle(param_dir, seed, chains):
         a = np.concatenate([a, ii[m]])
         b = np.concatenate([b, jj[m]])
         s += 4 * L
-    got = pdmd.pair_codes(sysd, a, b)
+    got = pdmd.pair_codes(sysd, a, b, species=species, chaptype2=ch2)
     ref = np.array([orc.ev_code(int(x), int(y)) for x, y in zip(a, b)])
     assert np.array_equal(got, ref)
 

@@ -34,10 +34,12 @@ from oracle import Oracle
 HERE = os.path.dirname(os.path.abspath(__file__))
 # (fixture, build flags of the reference harness that wrote it)
 CASES = [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}), ("ref_2species_noh", {"no_h": True}),
-         ("ref_2species_hb", {}), ("ref_2species_tie", {})]
+         ("ref_2species_hb", {}), ("ref_2species_tie", {}), ("ref_2species_ch2", {"chaptype2": True})]
 # coltypes of the worker's N-C resolution (main.F90:1487-1937, eventdyn.f)
 HB_TYPES = (14, 15, 16, 23, 24, 25, 26)
 LOG_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("t", "<f8")])
+# residues of the first species block of every fixture (2 chains x 8)
+L1_CHAINS = 16
 
 
 def block_digests(log, block=1000):
@@ -125,11 +127,19 @@ def test_ev_code(case):
     bad = np.argwhere(got != ref)
     assert bad.size == 0, f"{len(bad)} codes differ, first (a,b)={bad[0] + 1}: oracle {got[tuple(bad[0])]} ref {ref[tuple(bad[0])]}"
     # what each build flag removes: -Dno_hbs every code 15, -Dno_h the code
-    # 16 wells between chains (the same-chain ones read hp1/hp2 and stay)
+    # 16 wells between chains (the same-chain ones read hp1/hp2 and stay),
+    # -Dchaptype=2 every code but 1 inside a second-species chain and the
+    # 16 wells between two second-species chains
     same = fx["chnnum"][:, None] == fx["chnnum"][None, :]
     assert (15 in ref) != bool(flags.get("no_hbs"))
     assert ((ref == 16) & ~same).any() != bool(flags.get("no_h"))
     assert ((ref == 16) & same).any()
+    sp2 = np.arange(N) >= 4 * L1_CHAINS
+    inner2 = same & sp2[:, None] & sp2[None, :]
+    assert (ref[inner2] != 1).any() != bool(flags.get("chaptype2"))
+    cross2 = ~same & sp2[:, None] & sp2[None, :]
+    if flags.get("chaptype2"):
+        assert not ((ref == 16) & cross2).any()
 
 
 def test_neighbour_lists_in_reference_order(case):
@@ -276,5 +286,6 @@ def _checks_match(check, fx, name):
         rows = fx["chk_" + sec]
         want = [int((rows[:, 0] == k).sum()) for k in range(3)]
         assert [got["overlaps"], got["bonds_long"], got["bonds_short"]] == want, (name, sec, got, want)
-        assert (got["first_i"], got["first_j"]) == (int(rows[0, 1]), int(rows[0, 2])), (name, sec)
-        assert bool(fx["chk_" + sec + "_over"]), name
+        first = (int(rows[0, 1]), int(rows[0, 2])) if rows.shape[0] else (0, 0)
+        assert (got["first_i"], got["first_j"]) == first, (name, sec)
+        assert bool(fx["chk_" + sec + "_over"]) == (rows.shape[0] > 0), name
