@@ -246,14 +246,25 @@ def main():
 
     # batched: wavefronts per system (given, or autotuned after the warm-up),
     # warm-up, then K timed steps, each one launch over all systems
-    tune = None
     if args.waves > 0:
         eng.set_waves(args.waves)
         waves = args.waves
     for _ in range(args.warmup):
         eng.advance(ids, [args.per_step] * len(ids))
+    tuning = None
     if args.waves <= 0:
-        waves, tune = eng.autotune_waves(ids, args.tune_events)
+        # per system: wavefronts (1, 2, 4, the most that fit) and the list
+        # rebuild's coarse cell factor (2..4), each system timed by its own
+        # device clock; probe entries are ordinary committed entries
+        wsys, cfsys, rates = eng.autotune_systems(ids, args.tune_events)
+        waves = int(wsys.max())
+        tuning = {
+            "waves_per_system": {str(int(v)): int((wsys == v).sum()) for v in np.unique(wsys)},
+            "cell_factor_per_system": {str(int(v)): int((cfsys == v).sum()) for v in np.unique(cfsys)},
+            "trial_rates_sum_per_s": {str(t): round(float(rates[:, q].sum()), 1)
+                                      for q, t in enumerate(("1", "2", "4", "max"))},
+            "probe_entries_per_trial": args.tune_events,
+        }
     coll0 = sum(eng.stats(i).coll for i in ids)
     launch_ms = []
     t0 = time.perf_counter()
@@ -339,9 +350,10 @@ def main():
             "collisions_per_system_per_step": args.per_step,
             "collisions_per_system_timed": args.per_step * args.steps,
             "config_collisions_per_system": 10 ** 7,
-            "mode": f"batched: one persistent launch, one workgroup of {waves} wavefronts per system",
-            "waves_per_system": waves,
-            "autotune_rates": None if tune is None else {str(k): round(v, 1) for k, v in tune.items()},
+            "mode": f"batched: one persistent launch, one workgroup of {waves} wavefronts per system "
+                    "(each system limited to its own tuned count)",
+            "waves_per_launch": waves,
+            "autotune": tuning,
         },
         "plain_collisions_per_s": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(value / plain, 2),

@@ -68,6 +68,11 @@ typedef struct {
     double avg_up, avg_dn; /* mean up/down neighbour-list lengths (K_up, K_dn) */
     int64_t phase_cycles[32]; /* device cycles per engine phase; zero unless the
                                  library is built with -DPDMD_PHASE_TIMERS */
+    double device_ms;  /* wall-clock device time this system has spent in event-loop launches */
+    double rebuild_ms; /* ... of which in neighbour-list rebuilds (nbor.f + events.f at a sync) */
+    int64_t rebuilds;  /* the rebuilds rebuild_ms covers */
+    int32_t waves;       /* pair events per round this system is limited to (0: the context's setting) */
+    int32_t cell_factor; /* fine cells per coarse cell of its rebuild search */
 } pdmd_stats_t;
 
 typedef struct {
@@ -174,6 +179,25 @@ int pdmd_set_graph_launch(pdmd_ctx* ctx, int32_t on);
  * NULL) the committed entries per second of each trial (0 if skipped). */
 int pdmd_autotune_waves(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, int64_t probe_events,
                         int32_t* waves_out, double* rates_out);
+/* Per-system tuning, no counterpart in the reference (results do not depend
+ * on it): waves caps the pair events one round of this system runs side by
+ * side (1..8, 0 = the context's pdmd_set_waves); cell_factor (2..4, 0 =
+ * keep) is the coarse cell of its list rebuild's search in fine cells (the
+ * fine cells of main.F90:390 fix the list membership; the coarse grid only
+ * sets how many candidates each bead scans). */
+int pdmd_set_system_tuning(pdmd_ctx* ctx, int32_t sys_id, int32_t waves, int32_t cell_factor);
+/* Autotuner of both per system: advances the listed systems by
+ * probe_events entries each at 1, 2, 4 and the most wavefronts that fit
+ * (one launch each, the per-system device clock timing every system
+ * separately), then at cell factors 2, 3 and 4 (systems of more than 256
+ * beads; smaller ones search all pairs), and keeps for each system its
+ * fastest wavefront count and its cheapest rebuild; the context's launch
+ * width becomes the largest chosen.  Probe entries are committed like any
+ * others.  waves_out / cell_factor_out (nsys each) get the choices,
+ * rates_out (nsys x 4, may be NULL) each system's entries per second at
+ * the four wavefront trials (0 if skipped). */
+int pdmd_autotune_systems(pdmd_ctx* ctx, int32_t nsys, const int32_t* sys_ids, int64_t probe_events,
+                          int32_t* waves_out, int32_t* cell_factor_out, double* rates_out);
 /* Device time of the most recent pdmd_advance* launch (HIP events on the
  * context's stream), in milliseconds. */
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms);
@@ -189,6 +213,15 @@ int pdmd_stats(pdmd_ctx* ctx, int32_t sys_id, pdmd_stats_t* out);
  * type, event time); for parity checks against the CPU oracle. */
 int pdmd_log_enable(pdmd_ctx* ctx, int32_t sys_id, int32_t capacity);
 int pdmd_log_read(pdmd_ctx* ctx, int32_t sys_id, int32_t max, pdmd_event_t* out, int32_t* n);
+
+/* Test hook: the neighbour lists as the last rebuild left them (nbor.f's
+ * nb / na_npt and dnnab / nnabdn, header.f:21): n_up[k] and n_dn[k] the
+ * row lengths of bead k+1, up[k*cap + m] / dn[k*cap + m] its 1-based
+ * partners (rows in the device's order: by coarse cell then index, or by
+ * index up to 256 beads; the membership is the reference's); cap as
+ * pdmd_list_capacity returns.  Any pointer may be NULL. */
+int pdmd_list_capacity(pdmd_ctx* ctx, int32_t sys_id);
+int pdmd_lists(pdmd_ctx* ctx, int32_t sys_id, int32_t* n_up, int32_t* up, int32_t* n_dn, int32_t* dn);
 
 /* Test hook replacing core.f/bond.f/sqwel.f/sqshlder.f/nc_sqwel.f called
  * as events.f:28-47 does: tij (1e9 when none) and type for pairs (i, j),

@@ -557,6 +557,8 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.cap = H.cap;
     D.canon = H.canon;
     D.chaptype2 = H.topo.chaptype2 ? 1 : 0;
+    D.waves_cap = 0;
+    D.cq = 2;
     D.M = M;
     D.Mq = Mq;
     D.ncoarse = Mq * Mq * Mq;
@@ -828,6 +830,95 @@ int pdmd_autotune_waves(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, int64_t
     return 0;
 }
 
+int pdmd_set_system_tuning(pdmd_ctx* ctx, int32_t id, int32_t waves, int32_t cell_factor) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H) return -1;
+    if (waves < 0 || waves > 8) return fail("pdmd_set_system_tuning: waves must be 0 to 8");
+    if (cell_factor != 0 && (cell_factor < 2 || cell_factor > 4))
+        return fail("pdmd_set_system_tuning: cell factor must be 2 to 4 (0 keeps it)");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    H->dev.waves_cap = waves;
+    if (cell_factor) H->dev.cq = cell_factor;
+    HIPCHK(hipMemcpy(ctx->d_sys.p + (id - 1), &H->dev, sizeof(SysDev), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pdmd_autotune_systems(pdmd_ctx* ctx, int32_t nsys, const int32_t* ids, int64_t probe_events, int32_t* waves_out,
+                          int32_t* cell_factor_out, double* rates_out) {
+    if (!ctx || nsys <= 0 || !ids || probe_events <= 0 || !waves_out || !cell_factor_out)
+        return fail("pdmd_autotune_systems: bad arguments");
+    int max_n = 0, max_cap = 0;
+    for (int k = 0; k < nsys; ++k) {
+        HostSys* H = get_sys(ctx, ids[k]);
+        if (!H) return -1;
+        max_n = std::max(max_n, H->N);
+        max_cap = std::max(max_cap, H->cap);
+    }
+    const int top = engine_max_waves(max_n, engine_maxw(max_cap), ctx->lds_limit);
+    int cand[4] = {1, 2, 4, top > 4 ? top : 0};
+    std::vector<int64_t> nev(nsys, probe_events);
+    std::vector<SysState> st0(nsys), st1(nsys);
+    auto snap = [&](std::vector<SysState>& v) -> int {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (int k = 0; k < nsys; ++k)
+            HIPCHK(hipMemcpy(&v[k], get_sys(ctx, ids[k])->st.p, sizeof(SysState), hipMemcpyDeviceToHost));
+        return 0;
+    };
+    // wavefronts: every system at each candidate, timed by its own clock
+    std::vector<double> best(nsys, -1.0);
+    for (int k = 0; k < nsys; ++k) waves_out[k] = 1;
+    for (int q = 0; q < 4; ++q) {
+        for (int k = 0; k < nsys; ++k)
+            if (rates_out) rates_out[4 * k + q] = 0.0;
+        if (cand[q] == 0 || cand[q] > top) continue;
+        for (int k = 0; k < nsys; ++k)
+            if (int e = pdmd_set_system_tuning(ctx, ids[k], cand[q], 0)) return e;
+        ctx->waves = cand[q];
+        if (int e = snap(st0)) return e;
+        if (int e = pdmd_advance(ctx, nsys, ids, nev.data())) return e;
+        if (int e = snap(st1)) return e;
+        for (int k = 0; k < nsys; ++k) {
+            const double sec = (double)(st1[k].run_ticks - st0[k].run_ticks) / ((double)ctx->wall_khz * 1e3);
+            const double rate = sec > 0.0 ? (double)(st1[k].coll - st0[k].coll) / sec : 0.0;
+            if (rates_out) rates_out[4 * k + q] = rate;
+            if (rate > best[k]) {
+                best[k] = rate;
+                waves_out[k] = cand[q];
+            }
+        }
+    }
+    int wmax = 1;
+    for (int k = 0; k < nsys; ++k) wmax = std::max(wmax, waves_out[k]);
+    for (int k = 0; k < nsys; ++k)
+        if (int e = pdmd_set_system_tuning(ctx, ids[k], waves_out[k], 0)) return e;
+    ctx->waves = wmax;
+    // rebuild cell factor: mean rebuild time at each factor, per system
+    std::vector<double> rb(nsys, 1e300);
+    for (int k = 0; k < nsys; ++k) cell_factor_out[k] = get_sys(ctx, ids[k])->dev.cq;
+    bool any = false;
+    for (int k = 0; k < nsys; ++k) any |= get_sys(ctx, ids[k])->N > 256;
+    if (any)
+        for (int cf = 2; cf <= 4; ++cf) {
+            for (int k = 0; k < nsys; ++k)
+                if (int e = pdmd_set_system_tuning(ctx, ids[k], waves_out[k], cf)) return e;
+            if (int e = snap(st0)) return e;
+            if (int e = pdmd_advance(ctx, nsys, ids, nev.data())) return e;
+            if (int e = snap(st1)) return e;
+            for (int k = 0; k < nsys; ++k) {
+                const long long n = st1[k].rebuilds - st0[k].rebuilds;
+                if (get_sys(ctx, ids[k])->N <= 256 || n <= 0) continue;
+                const double per = (double)(st1[k].rebuild_ticks - st0[k].rebuild_ticks) / (double)n;
+                if (per < rb[k]) {
+                    rb[k] = per;
+                    cell_factor_out[k] = cf;
+                }
+            }
+        }
+    for (int k = 0; k < nsys; ++k)
+        if (int e = pdmd_set_system_tuning(ctx, ids[k], waves_out[k], cell_factor_out[k])) return e;
+    return 0;
+}
+
 int pdmd_last_launch_ms(pdmd_ctx* ctx, float* ms) {
     if (!ctx || !ms) return fail("pdmd_last_launch_ms: null argument");
     if (!ctx->timed) return fail("pdmd_last_launch_ms: no launch yet");
@@ -913,6 +1004,11 @@ int pdmd_stats(pdmd_ctx* ctx, int32_t id, pdmd_stats_t* out) {
     out->avg_up = (double)su / H->N;
     out->avg_dn = (double)sd / H->N;
     for (int k = 0; k < 32; ++k) out->phase_cycles[k] = st.phase[k];
+    out->device_ms = (double)st.run_ticks / (double)ctx->wall_khz;
+    out->rebuild_ms = (double)st.rebuild_ticks / (double)ctx->wall_khz;
+    out->rebuilds = st.rebuilds;
+    out->waves = H->dev.waves_cap;
+    out->cell_factor = H->dev.cq;
     return 0;
 }
 
@@ -956,6 +1052,32 @@ int pdmd_log_read(pdmd_ctx* ctx, int32_t id, int32_t max, pdmd_event_t* out, int
         }
     }
     *n = cnt;
+    return 0;
+}
+
+int pdmd_list_capacity(pdmd_ctx* ctx, int32_t id) {
+    HostSys* H = get_sys(ctx, id);
+    return H ? H->cap : -1;
+}
+
+int pdmd_lists(pdmd_ctx* ctx, int32_t id, int32_t* n_up, int32_t* up, int32_t* n_dn, int32_t* dn) {
+    HostSys* H = get_sys(ctx, id);
+    if (!H) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t N = (size_t)H->N, cap = (size_t)H->cap;
+    if (n_up) HIPCHK(hipMemcpy(n_up, H->nup.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+    if (n_dn) HIPCHK(hipMemcpy(n_dn, H->ndn.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+    // rows hold partner | static code << 24; the ABI gives 1-based partners
+    auto rows = [&](const int* src, int32_t* dst) -> int {
+        std::vector<int> r(N * cap);
+        HIPCHK(hipMemcpy(r.data(), src, sizeof(int) * N * cap, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < N * cap; ++k) dst[k] = (r[k] & 0xFFFFFF) + 1;
+        return 0;
+    };
+    if (up)
+        if (int e = rows(H->nb_up.p, up)) return e;
+    if (dn)
+        if (int e = rows(H->nb_dn.p, dn)) return e;
     return 0;
 }
 

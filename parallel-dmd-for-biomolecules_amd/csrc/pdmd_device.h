@@ -84,6 +84,9 @@ struct SysState {
     long long target;
     long long max_nb_seen;
     long long phase[32];  // device cycles per engine phase (PDMD_PHASE_TIMERS builds only)
+    long long run_ticks;      // wall_clock64 ticks this system spent in event-loop launches
+    long long rebuild_ticks;  // ... of which in list rebuilds (with events())
+    long long rebuilds;       // list rebuilds those ticks cover
 };
 
 struct LogRec {
@@ -95,7 +98,9 @@ struct LogRec {
 struct SysDev {
     int N, nres, cap, canon;
     int M, Mq, ncoarse, log_cap;  // fine cells per axis (interior), coarse cells per axis
-    int chaptype2, pad;           // built as -Dchaptype=2 (checkover.f:67 then skips bond windows)
+    int chaptype2;                // built as -Dchaptype=2 (checkover.f:67 then skips bond windows)
+    int waves_cap;                // most pair events per round for this system (0: the launch's wavefronts)
+    int cq, pad;                  // fine cells per coarse cell of the list rebuild's search (2..4)
     double width, half, setemp, avegtime, hdelr;
     unsigned long long seed;
     const DevTables* tab;

@@ -2600,7 +2600,10 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     const DevTables& T = tabs();
     const int lane = lane_id(), wave = wave_id(), nw = L.nw;
     const int tid = (int)threadIdx.x, nthr = kW * nw;
-    const int N = S.N, M = S.M, Mq = S.Mq, NC = S.ncoarse, cap = S.cap;
+    // coarse cells of cq >= 2 fine cells each (the last one takes the
+    // remainder): every fine cell within the n_wrap = 2 stencil of a bead's
+    // lies in the 27 coarse cells around the bead's own
+    const int N = S.N, M = S.M, cq = S.cq, Mq = M / cq, NC = Mq * Mq * Mq, cap = S.cap;
     PH_BEGIN(trl);
 #pragma unroll 4
     for (int k = tid; k < N; k += nthr) {
@@ -2612,7 +2615,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         cy = ((cy % M) + M) % M;
         cz = ((cz % M) + M) % M;
         gs1(S.cfine + (k), cx | (cy << 10) | (cz << 20));
-        const int qx = min(cx >> 1, Mq - 1), qy = min(cy >> 1, Mq - 1), qz = min(cz >> 1, Mq - 1);
+        const int qx = min(cx / cq, Mq - 1), qy = min(cy / cq, Mq - 1), qz = min(cz / cq, Mq - 1);
         gs1(S.ckey + (k), qx + Mq * (qy + Mq * qz));
     }
     for (int q = tid; q < NC; q += nthr) {
@@ -2687,7 +2690,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         const int fi = gl1(S.cfine + (ii));
         const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
         // the coarse cell, as the binning step derived it from the fine one
-        const int qx = min(ix >> 1, Mq - 1), qy = min(iy >> 1, Mq - 1), qz = min(iz >> 1, Mq - 1);
+        const int qx = min(ix / cq, Mq - 1), qy = min(iy / cq, Mq - 1), qz = min(iz / cq, Mq - 1);
         const Rec bi = gld(S.rec + (ii));
         const int4 oi = gld(S.ovr + (ii));
         int nr = 0;
@@ -3541,6 +3544,10 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
     // workgroups (launch_t0 is reset to all ones before a budgeted launch),
     // so that a workgroup dispatched late does not get a later deadline
     long long w0 = wall_clock64();
+    const long long t_begin = w0;
+    long long t_rebuild = 0, n_rebuild = 0;
+    // the system's own limit on pair events per round (its tuned wavefronts)
+    const int kcap = S.waves_cap > 0 && S.waves_cap < nw ? S.waves_cap : nw;
     if (budget > 0) {
         if (threadIdx.x == 0) {
             atomicMin(launch_t0, (unsigned long long)w0);
@@ -3556,7 +3563,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
             int K = 0;
             if (c.coll < c.target && c.status == 0 && !stop_out && !(budget > 0 && wall_clock64() - w0 > budget)) {
                 long long left = c.target - c.coll;
-                K = select_batch(L, c, N, N3, (int)min((long long)nw, left));
+                K = select_batch(L, c, N, N3, (int)min((long long)kcap, left));
             }
             if (lane == 0) {
                 B.K = K;
@@ -3583,7 +3590,12 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
                     __syncthreads();
                 else
                     wave_sync();
-                if (B.rebuild) rebuild(S, T, L, c);
+                if (B.rebuild) {
+                    const long long tr = wall_clock64();
+                    rebuild(S, T, L, c);
+                    t_rebuild += wall_clock64() - tr;
+                    ++n_rebuild;
+                }
                 if (wave == 0) {
                     sync_finish(S, L, c);
                     log_event(S, log_count, c.coll, e, -1, 0, k.t);
@@ -3682,6 +3694,9 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
     if (threadIdx.x == 0) {
         S.st->log_count = log_count;
         store_sc(S.st, c);
+        S.st->run_ticks += wall_clock64() - t_begin;
+        S.st->rebuild_ticks += t_rebuild;
+        S.st->rebuilds += n_rebuild;
     }
 }
 

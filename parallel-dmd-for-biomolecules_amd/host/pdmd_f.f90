@@ -31,6 +31,9 @@ module pdmd_f
     integer(c_int32_t) :: status, max_list, nbeads, pad
     real(c_double) :: avg_up, avg_dn
     integer(c_int64_t) :: phase_cycles(32)
+    real(c_double) :: device_ms, rebuild_ms
+    integer(c_int64_t) :: rebuilds
+    integer(c_int32_t) :: waves, cell_factor
   end type pdmd_stats_t
 
   type, bind(C) :: pdmd_obs_t

@@ -63,6 +63,11 @@ class Stats(ctypes.Structure):
         ("avg_up", ctypes.c_double),
         ("avg_dn", ctypes.c_double),
         ("phase_cycles", ctypes.c_int64 * 32),
+        ("device_ms", ctypes.c_double),
+        ("rebuild_ms", ctypes.c_double),
+        ("rebuilds", ctypes.c_int64),
+        ("waves", ctypes.c_int32),
+        ("cell_factor", ctypes.c_int32),
     ]
 
 
@@ -133,7 +138,8 @@ EXPORTS = [
     "pdmd_server_create", "pdmd_server_destroy", "pdmd_server_submit", "pdmd_server_step", "pdmd_server_pending",
     "pdmd_server_result", "pdmd_server_release", "pdmd_advance_budget_async", "pdmd_advance_to_output",
     "pdmd_set_waves", "pdmd_autotune_waves", "pdmd_check", "pdmd_set_graph_launch", "pdmd_rebuild",
-    "pdmd_observables_noalloc",
+    "pdmd_observables_noalloc", "pdmd_list_capacity", "pdmd_lists", "pdmd_set_system_tuning",
+    "pdmd_autotune_systems",
 ]
 
 _lib = None
@@ -173,6 +179,11 @@ def lib():
         L.pdmd_predict_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f64p, _i32p]
         L.pdmd_observables.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
         L.pdmd_observables_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Obs)]
+        L.pdmd_list_capacity.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.pdmd_set_system_tuning.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        L.pdmd_autotune_systems.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, ctypes.c_int64, _i32p, _i32p,
+                                            _f64p]
+        L.pdmd_lists.argtypes = [ctypes.c_void_p, ctypes.c_int32, _i32p, _i32p, _i32p, _i32p]
         L.pdmd_generate.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_uint64, _i32p, _i32p, _f64p, _f64p]
         L.pdmd_server_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
@@ -275,6 +286,19 @@ class Engine:
         _check(lib().pdmd_check(self._ctx, sid, None if keep is None else _p(keep, ctypes.c_double), ctypes.byref(out)))
         return {n: int(getattr(out, n)) for n, _ in Check._fields_}
 
+    def set_system_tuning(self, sid, waves=0, cell_factor=0):
+        _check(lib().pdmd_set_system_tuning(self._ctx, sid, waves, cell_factor))
+
+    def autotune_systems(self, ids, probe_events=3000):
+        """pdmd_autotune_systems: per-system wavefronts and rebuild cell
+        factor; returns (waves, cell_factor, rates[nsys, 4])."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        n = len(ids)
+        w, cf, r = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(4 * n)
+        _check(lib().pdmd_autotune_systems(self._ctx, n, _p(ids, ctypes.c_int32), int(probe_events),
+                                           _p(w, ctypes.c_int32), _p(cf, ctypes.c_int32), _p(r, ctypes.c_double)))
+        return w, cf, r.reshape(n, 4)
+
     def autotune_waves(self, ids, probe_events=2000):
         """Pick the wavefronts per system by timing short advances of the
         listed systems (pdmd_autotune_waves); returns (waves, {waves: rate})."""
@@ -355,6 +379,18 @@ class Engine:
         s = Stats()
         _check(lib().pdmd_stats(self._ctx, sid, ctypes.byref(s)))
         return s
+
+    def lists(self, sid):
+        """Neighbour rows as the last rebuild left them: (up, dn), lists of
+        1-based partner arrays per bead (pdmd_lists)."""
+        N = self.nbeads[sid]
+        cap = lib().pdmd_list_capacity(self._ctx, sid)
+        nu, nd = np.zeros(N, np.int32), np.zeros(N, np.int32)
+        up, dn = np.zeros(N * cap, np.int32), np.zeros(N * cap, np.int32)
+        _check(lib().pdmd_lists(self._ctx, sid, _p(nu, ctypes.c_int32), _p(up, ctypes.c_int32),
+                                _p(nd, ctypes.c_int32), _p(dn, ctypes.c_int32)))
+        up, dn = up.reshape(N, cap), dn.reshape(N, cap)
+        return [up[k, :nu[k]].copy() for k in range(N)], [dn[k, :nd[k]].copy() for k in range(N)]
 
     def log_enable(self, sid, capacity):
         _check(lib().pdmd_log_enable(self._ctx, sid, capacity))

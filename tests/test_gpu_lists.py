@@ -1,9 +1,11 @@
 """Neighbour lists on the device against the oracle's (nbor.f:23-137
-restated): right after set-up, the longest up or down row and the total row
-lengths equal the oracle's; after 20000 entries (several list rebuilds at
-the syncs) the totals still do.  Systems of up to 256 beads build their lists
-with the all-pairs stencil filter, larger ones through the cell sort; both
-are covered, at 1 and 4 wavefronts."""
+restated, pinned by the reference's own lists in test_oracle_reference.py):
+right after set-up every bead's up and down rows hold the oracle's partners
+(compared as sets: the device orders rows by cell, the reference by its
+cell traversal), the longest row and the totals agree; after 20000 entries
+(several list rebuilds at the syncs) the rows agree again.  Systems of up
+to 256 beads build their lists with the all-pairs stencil filter, larger
+ones through the cell sort; both are covered, at 1 and 4 wavefronts."""
 import numpy as np
 import pytest
 
@@ -18,6 +20,13 @@ def _oracle_rows(orc, n):
     up = np.array([orc.nb_up(i).size for i in range(1, n + 1)])
     dn = np.array([orc.nb_dn(i).size for i in range(1, n + 1)])
     return up, dn
+
+
+def _same_rows(eng, sid, orc, n):
+    gup, gdn = eng.lists(sid)
+    for i in range(1, n + 1):
+        assert np.array_equal(np.sort(gup[i - 1]), np.sort(orc.nb_up(i))), f"up row of bead {i}"
+        assert np.array_equal(np.sort(gdn[i - 1]), np.sort(orc.nb_dn(i))), f"down row of bead {i}"
 
 
 @pytest.mark.parametrize("chains,density,tstar", [
@@ -40,6 +49,7 @@ def test_list_lengths_match_oracle(param_dir, chains, density, tstar, waves):
         assert st.status == 0
         assert st.max_list == max(up.max(), dn.max())
         assert round(st.avg_up * n) == up.sum() and round(st.avg_dn * n) == dn.sum()
+        _same_rows(eng, sid, orc, n)
         eng.advance([sid], [20000])
         orc.run(20000)
         st = eng.stats(sid)
@@ -47,5 +57,6 @@ def test_list_lengths_match_oracle(param_dir, chains, density, tstar, waves):
         assert st.nupdates == orc.stats().nupdates
         up, dn = _oracle_rows(orc, n)
         assert round(st.avg_up * n) == up.sum() and round(st.avg_dn * n) == dn.sum()
+        _same_rows(eng, sid, orc, n)
     finally:
         eng.close()
