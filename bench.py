@@ -108,17 +108,29 @@ def cpu_baseline(systems, seeds, per_sys, param_dir, workers):
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def measured_bytes_per_entry():
-    """HBM bytes per committed entry of k_run from the FETCH_SIZE/WRITE_SIZE
-    counter passes summarised in profiles/ (None if absent)."""
-    path = os.path.join(PROFILES, "k_run_uneven64_hbm_counters.md")
+def profile_figure(name, key):
+    """A figure (`key: value` line) of a counter summary in profiles/
+    (None if absent)."""
     try:
-        for line in open(path):
-            if line.startswith("bytes_per_entry:"):
+        for line in open(os.path.join(PROFILES, name)):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def measured_bytes_per_entry():
+    """HBM bytes per committed entry of k_run from the FETCH_SIZE/WRITE_SIZE
+    counter passes."""
+    return profile_figure("k_run_uneven64_hbm_counters.md", "bytes_per_entry")
+
+
+def measured_fp64_per_entry():
+    """FP64 operations per committed entry of k_run from the FP64 VALU
+    instruction counters, weighted by the measured VALU lane utilisation
+    (an FMA counts two)."""
+    return profile_figure("k_run_uneven64_fp64_counters.md", "fp64_flops_per_entry")
 
 
 def offered_load(rate, lo, hi):
@@ -261,7 +273,9 @@ def main():
         tuning = {
             "waves_per_system": {str(int(v)): int((wsys == v).sum()) for v in np.unique(wsys)},
             "cell_factor_per_system": {str(int(v)): int((cfsys == v).sum()) for v in np.unique(cfsys)},
-            "trial_rates_sum_per_s": {str(t): round(float(rates[:, q].sum()), 1)
+            # a trial the tuner skipped (the most that fit equals a probed
+            # count) reads as null
+            "trial_rates_sum_per_s": {str(t): (round(float(rates[:, q].sum()), 1) if rates[:, q].any() else None)
                                       for q, t in enumerate(("1", "2", "4", "max"))},
             "probe_entries_per_trial": args.tune_events,
         }
@@ -283,8 +297,11 @@ def main():
     ev_per_launch = args.per_step * len(ids)
     avg_launch_s = float(np.mean(launch_ms)) / 1e3
     achieved = ev_per_launch * bpe / avg_launch_s / 1e9
-    # the root solves: 2 (K_up + K_dn) predictions per committed entry
-    fp64_gflops = ev_per_launch * 2.0 * (k_up + k_dn) * FLOPS_PER_PREDICTION / avg_launch_s / 1e9
+    # FP64 rate: the counted FP64 operations per entry over this run's
+    # launch time, beside the algorithmic 2 (K_up + K_dn) root solves
+    fp64_alg_gflops = ev_per_launch * 2.0 * (k_up + k_dn) * FLOPS_PER_PREDICTION / avg_launch_s / 1e9
+    fpe_meas = measured_fp64_per_entry()
+    fp64_gflops = None if fpe_meas is None else ev_per_launch * fpe_meas / avg_launch_s / 1e9
     bpe_meas = measured_bytes_per_entry()
 
     # plain: the same systems one at a time
@@ -374,13 +391,31 @@ def main():
             "k_dn": round(k_dn, 2),
             "avg_launch_ms": round(avg_launch_s * 1e3, 3),
         },
-        "fp64_root_solves": {
-            "achieved": round(fp64_gflops, 3),
+        "fp64": {
+            "kernel": "k_run",
+            "achieved": None if fp64_gflops is None else round(fp64_gflops, 3),
             "peak": 78600.0,
             "unit": "GFLOP/s",
-            "frac": fp64_gflops / 78600.0,
-            "flops_per_prediction": FLOPS_PER_PREDICTION,
-            "predictions_per_entry": round(2.0 * (k_up + k_dn), 2),
+            "frac": None if fp64_gflops is None else fp64_gflops / 78600.0,
+            "flops_per_entry_counted": fpe_meas,
+            "source": "FP64 VALU instruction counters x 64 x VALU lane utilisation (FMA = 2) per committed "
+                      "entry, over this run's launch time",
+            "algorithmic_root_solves": {
+                "achieved": round(fp64_alg_gflops, 3),
+                "frac": fp64_alg_gflops / 78600.0,
+                "flops_per_prediction": FLOPS_PER_PREDICTION,
+                "predictions_per_entry": round(2.0 * (k_up + k_dn), 2),
+            },
+        },
+        "list_and_predictor_kernels": {
+            "source": "standalone dispatches on this workload (kernel_report.py under rocprofv3), "
+                      "profiles/k_rebuild_uneven64_counters.md and profiles/k_predict_uneven64_counters.md",
+            "unit": "GB/s of algorithmic bytes",
+            "k_rebuild_lists": profile_figure("k_rebuild_uneven64_counters.md", "lists_algorithmic_gbps"),
+            "k_rebuild_events": profile_figure("k_rebuild_uneven64_counters.md", "events_algorithmic_gbps"),
+            "k_predict_largest": profile_figure("k_predict_uneven64_counters.md", "predict_largest_algorithmic_gbps"),
+            "k_predict_largest_fp64_frac": profile_figure("k_predict_uneven64_counters.md",
+                                                          "predict_largest_fp64_frac"),
         },
         "cpu_baseline": cpu,
         "setup_s": round(gen_s + setup_s, 2),

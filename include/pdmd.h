@@ -66,13 +66,16 @@ typedef struct {
     int32_t status, max_list;
     int32_t nbeads, pad;
     double avg_up, avg_dn; /* mean up/down neighbour-list lengths (K_up, K_dn) */
-    int64_t phase_cycles[32]; /* device cycles per engine phase; zero unless the
+    int64_t phase_cycles[40]; /* device cycles and counts per engine phase; zero unless the
                                  library is built with -DPDMD_PHASE_TIMERS */
     double device_ms;  /* wall-clock device time this system has spent in event-loop launches */
     double rebuild_ms; /* ... of which in neighbour-list rebuilds (nbor.f + events.f at a sync) */
     int64_t rebuilds;  /* the rebuilds rebuild_ms covers */
     int32_t waves;       /* pair events per round this system is limited to (0: the context's setting) */
     int32_t cell_factor; /* fine cells per coarse cell of its rebuild search */
+    int32_t cells_axis;  /* fine cells per box edge (main.F90:390, interior cells); the rebuild
+                            search grid has (cells_axis / cell_factor)^3 cells */
+    int32_t pad2;
 } pdmd_stats_t;
 
 typedef struct {

@@ -755,7 +755,7 @@ static void add_tbin(orc_sys* S, int i) {
     int j = (int)((S->tim[i] + S->tbin_off) / S->sortsize) + 1;
     if (j > NUMBIN) j = NUMBIN; /* -Ddebugging clamp */
     if (j < 0) {
-        S->err = -10;
+        if (!S->err) S->err = -10; /* a negative time (-12) reported first keeps its code */
         j = 0;
     }
     S->tlinks[i] = S->bin[j];

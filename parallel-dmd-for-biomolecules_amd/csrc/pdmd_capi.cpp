@@ -17,6 +17,7 @@
 #include "pdmd_device.h"
 #include "pdmd_engine.h"
 #include "pdmd_host.h"
+#include "pdmd_internal.h"
 
 namespace pdmd {
 namespace {
@@ -429,6 +430,11 @@ const Params* shipped_params() {
 }
 
 }  // namespace
+
+struct PreparedSystem {
+    Prep pp;
+};
+
 }  // namespace pdmd
 
 using namespace pdmd;
@@ -508,8 +514,33 @@ int64_t pdmd_system_bytes_ctx(pdmd_ctx* ctx, const pdmd_system_desc* d) {
 
 int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     if (!ctx || !sys_id) return fail("pdmd_system_add: null argument");
-    Prep pp;
-    if (int e = prepare_system(ctx->raw, d, pp)) return e;
+    PreparedSystem ps;
+    if (int e = prepare_system(ctx->raw, d, ps.pp)) return e;
+    return prepared_add(ctx, &ps, sys_id);
+}
+
+}  // extern "C"
+
+namespace pdmd {
+
+PreparedSystem* prepared_new(pdmd_ctx* ctx, const pdmd_system_desc* d) {
+    if (!ctx) {
+        fail("pdmd_system_add: null context");
+        return nullptr;
+    }
+    auto ps = std::make_unique<PreparedSystem>();
+    if (prepare_system(ctx->raw, d, ps->pp)) return nullptr;
+    return ps.release();
+}
+
+void prepared_free(PreparedSystem* p) { delete p; }
+
+int64_t prepared_bytes(const PreparedSystem* p) { return device_bytes(p->pp); }
+
+int prepared_add(pdmd_ctx* ctx, PreparedSystem* ps, int32_t* sys_id) {
+    if (!ctx || !ps || !sys_id || !ps->pp.hs) return fail("pdmd_system_add: null argument");
+    Prep pp = std::move(ps->pp);
+    ps->pp = Prep{};
     // an advance may still be running on the context's stream: it reads the
     // descriptor table and the id buffer that the start-up below rewrites
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -568,7 +599,7 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     D.setemp = setemp;
     D.avegtime = 0.00005 / std::sqrt(setemp);
     D.hdelr = H.cut.hdelr;
-    D.seed = d->seed;
+    D.seed = H.seed;
     D.tab = H.tab.p;
     D.rec = H.rec.p;
     D.bw = H.bw.p;
@@ -626,6 +657,10 @@ int pdmd_system_add(pdmd_ctx* ctx, const pdmd_system_desc* d, int32_t* sys_id) {
     *sys_id = id + 1;
     return 0;
 }
+
+}  // namespace pdmd
+
+extern "C" {
 
 
 static HostSys* get_sys(pdmd_ctx* ctx, int32_t id) {
@@ -1003,12 +1038,13 @@ int pdmd_stats(pdmd_ctx* ctx, int32_t id, pdmd_stats_t* out) {
     }
     out->avg_up = (double)su / H->N;
     out->avg_dn = (double)sd / H->N;
-    for (int k = 0; k < 32; ++k) out->phase_cycles[k] = st.phase[k];
+    for (int k = 0; k < kPhaseSlots; ++k) out->phase_cycles[k] = st.phase[k];
     out->device_ms = (double)st.run_ticks / (double)ctx->wall_khz;
     out->rebuild_ms = (double)st.rebuild_ticks / (double)ctx->wall_khz;
     out->rebuilds = st.rebuilds;
     out->waves = H->dev.waves_cap;
     out->cell_factor = H->dev.cq;
+    out->cells_axis = H->dev.M;
     return 0;
 }
 

@@ -73,6 +73,9 @@ struct DevTables {
     double epsilon1;
 };
 
+// Slots of SysState::phase (pdmd_stats_t::phase_cycles).
+constexpr int kPhaseSlots = 40;
+
 // Mutable per-system scalars (the reference's module GLOBAL scalars).
 struct SysState {
     double tfalse, t, interval, interval_max, t_fact, n_forced, sortsize, tbin_off;
@@ -83,7 +86,7 @@ struct SysState {
     int nbin, status, log_count, pad1;
     long long target;
     long long max_nb_seen;
-    long long phase[32];  // device cycles per engine phase (PDMD_PHASE_TIMERS builds only)
+    long long phase[kPhaseSlots];  // device cycles and counts per engine phase (PDMD_PHASE_TIMERS builds only)
     long long run_ticks;      // wall_clock64 ticks this system spent in event-loop launches
     long long rebuild_ticks;  // ... of which in list rebuilds (with events())
     long long rebuilds;       // list rebuilds those ticks cover

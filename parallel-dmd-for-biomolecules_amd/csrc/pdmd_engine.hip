@@ -56,12 +56,14 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(th
 // the second, 26 new keys up to the third, 27 the round's rescans and merge,
 // 28 rounds cut by a footprint conflict or an event that cannot speculate,
 // 29 rounds cut by the key order; round selection: 30 the pops, 31 the
-// scalar copies after them.
+// scalar copies after them; rescheduling batches: 32 pair predictions made
+// in them, 33 their 64-lane passes (32 / (64 * 33) is their lane occupancy).
 enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
        kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
        kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20,
        kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23, kPhBMark = 24, kPhBCheck = 25, kPhBKeys = 26,
-       kPhBTail = 27, kPhNCutMark = 28, kPhNCutKey = 29, kPhSelPop = 30, kPhSelCopy = 31 };
+       kPhBTail = 27, kPhNCutMark = 28, kPhNCutKey = 29, kPhSelPop = 30, kPhSelCopy = 31, kPhNPred = 32,
+       kPhNPredPass = 33 };
 #ifdef PDMD_PHASE_TIMERS
 #define PH_BEGIN(v) long long v = clock64()
 #define PH_END(S, k, v)                                                                                  \
@@ -589,7 +591,8 @@ __device__ inline int calc_bin(const Sc& c, double tim, bool uncond, int& status
     int j = (int)((tim + c.tbin_off) / c.sortsize) + 1;
     if (j > kNumbin) j = kNumbin;  // -Ddebugging clamp, add_tbin.f:20-23
     if (j < 0) {
-        status = kErrBin;
+        // a negative time already reported (events.f:59-73 stops there) keeps its status
+        if (!status) status = kErrBin;
         return -1;
     }
     // inserted behind the scan position: the reference's bucket scan
@@ -1205,6 +1208,8 @@ __device__ void reschedule_generic(const SysDev&, const DevTables&, const Lds&, 
     wave_sync_mem();
     PH_END(S, kPhLoads, tph);
     // ---- candidate predictions of the full recomputes of i and j
+    PH_ADDN(S, kPhNPred, min(ncand, L.maxw));
+    PH_ADDN(S, kPhNPredPass, (ncand + kW - 1) / kW);
     double bt0 = 1e300, bt1 = 1e300;
     int bp0 = INT_MAX, bp1 = INT_MAX, bm0 = -1, bm1 = -1, bty0 = -1, bty1 = -1;
     for (int base = 0; base < ncand; base += kW) {
@@ -1738,6 +1743,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     PH_END(S, kPhLoads, tph);
     // ---- the prediction batch (round 3: the batched beads' partners, then
     //      the single-pair predictions)
+    PH_ADDN(S, kPhNPred, ntot);
+    PH_ADDN(S, kPhNPredPass, (ntot + kW - 1) / kW);
     for (int base = 0; base < ntot; base += kW) {
         const int q = base + lane;
         if (q < ntot) {
@@ -3024,13 +3031,12 @@ __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
     rescan_all(L, N + 3);
 }
 
-// Wrap, lists and full reschedule; all wavefronts of the system take part.
-__device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
+// Wrap and lists; all wavefronts of the system take part.
+__device__ void wrap_and_lists(const SysDev&, const Lds&, Sc&) {
     const Lds& L = ldsv();
 
     const SysDev& S = sdev();
 
-    const DevTables& T = tabs();
     Sc& c = *L.msc();
 
     const int tid = (int)threadIdx.x, nthr = kW * L.nw;
@@ -3044,11 +3050,20 @@ __device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
         gst(S.old_r + (k), make_double4(b.x, b.y, b.z, 0.0));
     }
     wg_sync(L.nw);
-    PH_BEGIN(trb);
     if (S.N <= kDenseN)
         rebuild_lists_dense(S, L, c);
     else
         rebuild_lists(S, L, c);
+}
+
+// Wrap, lists and full reschedule; all wavefronts of the system take part.
+__device__ void rebuild(const SysDev&, const DevTables&, const Lds&, Sc&) {
+    const Lds& L = ldsv();
+    const SysDev& S = sdev();
+    const DevTables& T = tabs();
+    Sc& c = *L.msc();
+    PH_BEGIN(trb);
+    wrap_and_lists(S, L, c);
     PH_END(S, kPhLists, trb);
     events_all(S, T, L, c);
     PH_END(S, kPhEvents, trb);
@@ -3349,13 +3364,10 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_setup(const SysDev* __rest
     if (threadIdx.x == 0) store_sc(S.st, c);
 }
 
-// A list update forced at the current time (pdmd_rebuild): the sync
-// pseudo-event's shift of the time origin and its update branch
-// (main.F90:1126-1187), then the lists and events() on every wavefront of
-// the system's workgroup.  No displacement test, no rate adaptation and no
-// new sync entry: the pending one keeps its shifted time.
-__global__ __launch_bounds__(64 * kSetupWaves) void k_rebuild(const SysDev* __restrict__ sys, const int* ids,
-                                                              int maxw) {
+// Workgroup entry of the rebuild kernels: the system's descriptor, the work
+// areas and the tables into LDS; the first lane loads the master scalars
+// (each wavefront's own copy is not loaded here).
+__device__ void enter_system(const SysDev* __restrict__ sys, const int* ids, int maxw) {
     const int nw = (int)(blockDim.x >> 6), wave = wave_id();
     if (threadIdx.x == 0) *reinterpret_cast<SysDev*>(smem + kSdOff) = sys[ids[blockIdx.x]];
     __syncthreads();
@@ -3363,13 +3375,24 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_rebuild(const SysDev* __re
     if (lane_id() == 0)
         *reinterpret_cast<Lds*>(smem + kLdOff + wave * kLdStride) = make_lds(S, maxw, 0, nw, wave);
     __syncthreads();
-    const Lds& L = ldsv();
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
+    if (threadIdx.x == 0) load_sc(S.st, *ldsv().msc());
     __syncthreads();
-    const DevTables& T = tabs();
-    Sc& c = *L.msc();  // the master scalars (each wavefront's own copy is not loaded here)
-    if (threadIdx.x == 0) load_sc(S.st, c);
-    __syncthreads();
+}
+
+// A list update forced at the current time (pdmd_rebuild), in two
+// dispatches so that the list build and the full reschedule are timed and
+// counted apart.  This first one runs the sync pseudo-event's shift of the
+// time origin and its update branch (main.F90:1126-1187) up to the new
+// lists, on every wavefront of the system's workgroup.  No displacement
+// test, no rate adaptation and no new sync entry: the pending one keeps its
+// shifted time.
+__global__ __launch_bounds__(64 * kSetupWaves) void k_rebuild_lists(const SysDev* __restrict__ sys, const int* ids,
+                                                                    int maxw) {
+    enter_system(sys, ids, maxw);
+    const SysDev& S = sdev();
+    const Lds& L = ldsv();
+    Sc& c = *L.msc();
     const int N = S.N;
     const double tf = c.tfalse;
     __syncthreads();  // every wavefront holds tf before the first lane moves the origin
@@ -3395,7 +3418,20 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_rebuild(const SysDev* __re
     }
     __threadfence_block();
     __syncthreads();
-    rebuild(S, T, L, c);
+    wrap_and_lists(S, L, c);
+    __syncthreads();
+    if (threadIdx.x == 0) store_sc(S.st, c);
+}
+
+// The second dispatch of a forced list update: events() over the new lists
+// (events.f), every bead's next event and the pseudo-events' bins.
+__global__ __launch_bounds__(64 * kSetupWaves) void k_rebuild_events(const SysDev* __restrict__ sys, const int* ids,
+                                                                     int maxw) {
+    enter_system(sys, ids, maxw);
+    const SysDev& S = sdev();
+    const Lds& L = ldsv();
+    Sc& c = *L.msc();
+    events_all(S, tabs(), L, c);
     __syncthreads();
     if (threadIdx.x == 0) store_sc(S.st, c);
 }
@@ -4003,7 +4039,10 @@ hipError_t engine_rebuild(const SysDev* d_sys, const int* d_ids, int n, int max_
     int nw = kSetupWaves;
     while (nw > 1 && engine_lds_bytes(nblk_for(max_n), maxw, 0, nw) > lds_limit) --nw;
     const size_t lds = engine_lds_bytes(nblk_for(max_n), maxw, 0, nw);
-    hipLaunchKernelGGL(k_rebuild, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
+    hipLaunchKernelGGL(k_rebuild_lists, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rebuild_events, dim3(n), dim3(kW * nw), lds, s, d_sys, d_ids, maxw);
     return hipGetLastError();
 }
 // clears the start word of a budgeted launch (a kernel rather than a memset,
@@ -4128,7 +4167,10 @@ hipError_t engine_prepare(size_t lds_limit) {
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)k_rebuild, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+    e = hipFuncSetAttribute((const void*)k_rebuild_lists, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_rebuild_events, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_limit);
 }
 
 }  // namespace pdmd

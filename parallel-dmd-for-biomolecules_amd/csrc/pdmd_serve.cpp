@@ -1,12 +1,15 @@
 // pdmd_serve.cpp -- continuous-batching request loop over one engine context.
 //
 // Trajectory requests (a PRIME20 system and a number of calendar entries to
-// commit) queue on the host.  Each step waits for the launch in flight,
+// commit) queue on the host.  Each step first prepares the host half of the
+// start-up (topology, H-bond restore, tables) of the requests at the head of
+// the queue while the launch in flight still runs, then waits for it,
 // retires the requests it finished (final state downloaded, device memory
 // freed), admits queued requests that fit the device-memory budget (first
 // fit in arrival order, so a large request does not block smaller ones
-// behind it), and launches the next quantum for every resident request in
-// one persistent launch.  A request's trajectory does not depend on which
+// behind it; a prepared request needs only its uploads and start-up kernel
+// here), and launches the next quantum for every resident request in one
+// persistent launch.  A request's trajectory does not depend on which
 // others share its launches or where the quanta fall: each system has its
 // own wavefront, calendar and random streams.
 #include <chrono>
@@ -19,6 +22,7 @@
 
 #include "../../include/pdmd.h"
 #include "pdmd_host.h"
+#include "pdmd_internal.h"
 
 namespace {
 
@@ -29,12 +33,22 @@ double now_s() {
 
 enum ReqState { kQueued = 0, kResident = 1, kDone = 2, kFailed = 3 };
 
+// queued requests whose host half of the start-up is kept ready at most
+// (their host copies of the system's tables and records)
+constexpr int kPrepAhead = 16;
+
+struct PrepFree {
+    void operator()(pdmd::PreparedSystem* p) const { pdmd::prepared_free(p); }
+};
+using PrepPtr = std::unique_ptr<pdmd::PreparedSystem, PrepFree>;
+
 struct Req {
     int64_t id = 0;
     // owned copies of the description's arrays
     std::vector<int32_t> chain_len, species, restype, hp, bptnr;
     std::vector<double> sv;
     pdmd_system_desc desc{};
+    PrepPtr prep;  // host half of the start-up, ready for admission
     int64_t ncoll = 0, done = 0, bytes = 0, coll0 = 0;
     int32_t sys = -1;
     int state = kQueued, status = 0;
@@ -58,6 +72,7 @@ struct pdmd_server {
     std::vector<int64_t> inc;  // increments of the launch in flight, per resident request
     bool inflight = false;
     int64_t committed = 0;     // calendar entries committed by finished launches
+    int nprep = 0;             // queued requests holding a prepared start-up
 };
 
 namespace {
@@ -99,6 +114,16 @@ int retire(pdmd_server* s, int32_t* ndone) {
     return 0;
 }
 
+// host halves of the start-up for the head of the queue (no device call)
+void prepare_ahead(pdmd_server* s) {
+    for (Req* r : s->queue) {
+        if (s->nprep >= kPrepAhead) break;
+        if (r->prep) continue;
+        r->prep.reset(pdmd::prepared_new(s->ctx, &r->desc));
+        if (r->prep) ++s->nprep;
+    }
+}
+
 int admit(pdmd_server* s) {
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Req* r = *it;
@@ -108,7 +133,13 @@ int admit(pdmd_server* s) {
             continue;
         }
         int32_t sid = -1;
-        int rc = pdmd_system_add(s->ctx, &r->desc, &sid);
+        int rc;
+        if (r->prep) {
+            rc = pdmd::prepared_add(s->ctx, r->prep.get(), &sid);
+            r->prep.reset();
+            --s->nprep;
+        } else
+            rc = pdmd_system_add(s->ctx, &r->desc, &sid);
         if (rc) {
             r->state = kFailed;
             r->status = rc;
@@ -200,8 +231,16 @@ int pdmd_server_submit(pdmd_server* s, const pdmd_system_desc* d, int64_t ncoll,
     r->desc.hp = r->hp.data();
     r->desc.sv = r->sv.data();
     r->desc.bptnr = d->bptnr ? r->bptnr.data() : nullptr;
-    r->bytes = pdmd_system_bytes_ctx(s->ctx, &r->desc);
-    if (r->bytes <= 0 || r->bytes > s->budget) return serve_fail("pdmd_server_submit: request exceeds the memory budget");
+    // the host half of the start-up sizes the request; it is kept for
+    // admission while few queued requests hold one
+    PrepPtr prep(pdmd::prepared_new(s->ctx, &r->desc));
+    if (!prep) return -1;
+    r->bytes = pdmd::prepared_bytes(prep.get());
+    if (r->bytes > s->budget) return serve_fail("pdmd_server_submit: request exceeds the memory budget");
+    if (s->nprep < kPrepAhead) {
+        r->prep = std::move(prep);
+        ++s->nprep;
+    }
     r->ncoll = ncoll;
     r->id = s->next_id++;
     r->t_submit = now_s() - s->t0;
@@ -214,6 +253,7 @@ int pdmd_server_submit(pdmd_server* s, const pdmd_system_desc* d, int64_t ncoll,
 int pdmd_server_step(pdmd_server* s, int32_t* ncompleted) {
     if (!s) return -1;
     if (ncompleted) *ncompleted = 0;
+    prepare_ahead(s);  // overlaps the launch in flight
     if (s->inflight) {
         int rc = retire(s, ncompleted);
         if (rc) return rc;

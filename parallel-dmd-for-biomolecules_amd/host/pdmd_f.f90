@@ -30,10 +30,10 @@ module pdmd_f
     real(c_double) :: t, tfalse, interval, interval_max
     integer(c_int32_t) :: status, max_list, nbeads, pad
     real(c_double) :: avg_up, avg_dn
-    integer(c_int64_t) :: phase_cycles(32)
+    integer(c_int64_t) :: phase_cycles(40)
     real(c_double) :: device_ms, rebuild_ms
     integer(c_int64_t) :: rebuilds
-    integer(c_int32_t) :: waves, cell_factor
+    integer(c_int32_t) :: waves, cell_factor, cells_axis, pad2
   end type pdmd_stats_t
 
   type, bind(C) :: pdmd_obs_t

@@ -62,12 +62,14 @@ class Stats(ctypes.Structure):
         ("pad", ctypes.c_int32),
         ("avg_up", ctypes.c_double),
         ("avg_dn", ctypes.c_double),
-        ("phase_cycles", ctypes.c_int64 * 32),
+        ("phase_cycles", ctypes.c_int64 * 40),
         ("device_ms", ctypes.c_double),
         ("rebuild_ms", ctypes.c_double),
         ("rebuilds", ctypes.c_int64),
         ("waves", ctypes.c_int32),
         ("cell_factor", ctypes.c_int32),
+        ("cells_axis", ctypes.c_int32),
+        ("pad2", ctypes.c_int32),
     ]
 
 

@@ -20,7 +20,7 @@ if HERE not in sys.path:
     sys.path.insert(0, HERE)
 import pdmd  # noqa: E402
 
-# slot: (name, parent slot or None); slots 13/14 and 21-23, 28-29 are counts
+# slot: (name, parent slot or None); slots 13/14, 21-23, 28-29 and 32-33 are counts
 PHASES = {
     0: ("select", None), 30: ("select: pops", 0), 31: ("select: scalar copies", 0),
     1: ("pair event before rescheduling", None), 11: ("resched: list rows + work list", None),
@@ -65,7 +65,7 @@ def main():
                 raise SystemExit(f"{name}: status {s1.status}")
         finally:
             e.close()
-        d = [s1.phase_cycles[k] - s0.phase_cycles[k] for k in range(32)]
+        d = [s1.phase_cycles[k] - s0.phase_cycles[k] for k in range(40)]
         if not any(d[k] for k in PHASES):
             raise SystemExit("the loaded library has no phase timers")
         cols.append((d, ms))
@@ -82,7 +82,8 @@ def main():
     for (name, *_), (d, ms) in zip(CASES, cols):
         print(f"{name}: {args.events / (ms / 1e3):.0f} entries/s at W={args.waves}; rounds of two or more events "
               f"{d[21]}, popped {d[22]}, committed {d[23]}, cut by footprints {d[28]}, cut by key order {d[29]}; "
-              f"list rebuilds {d[13]}")
+              f"list rebuilds {d[13]}; rescheduling batches: {d[32]} pair predictions in {d[33]} 64-lane passes "
+              f"(lane occupancy {d[32] / max(64 * d[33], 1):.3f}, wavefront 0)")
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@
   violation by the oracle's checkover.f restatement), deterministic per seed,
   at the requested density and temperature.
 """
+import ctypes
 import os
 import re
 
@@ -139,6 +140,34 @@ def test_header_is_plain_c99(tmp_path):
                         "-I", os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "use"),
                         "-L", pkg, "-lpdmd", "-Wl,-rpath," + pkg], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_ctypes_structures_match_the_c_layout(tmp_path):
+    """Every structure pdmd.py mirrors has the C header's size and field
+    offsets (a mismatch would let the library write past a Python buffer)."""
+    import subprocess
+
+    pairs = [(pdmd.SystemDesc, "pdmd_system_desc"), (pdmd.Stats, "pdmd_stats_t"),
+             (pdmd.RequestInfo, "pdmd_request_info_t"), (pdmd.Event, "pdmd_event_t"),
+             (pdmd.Obs, "pdmd_obs_t"), (pdmd.Check, "pdmd_check_t")]
+    lines = []
+    for py, c in pairs:
+        lines.append(f'  printf("{c} size %zu\\n", sizeof({c}));')
+        for name, *_ in py._fields_:
+            lines.append(f'  printf("{c} {name} %zu\\n", offsetof({c}, {name}));')
+    src = tmp_path / "layout.c"
+    src.write_text('#include "pdmd.h"\n#include <stddef.h>\n#include <stdio.h>\nint main(void) {\n' +
+                   "\n".join(lines) + "\n  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln}
+    for py, c in pairs:
+        assert got[(c, "size")] == ctypes.sizeof(py), c
+        for name, *_ in py._fields_:
+            assert got[(c, name)] == getattr(py, name).offset, (c, name)
 
 
 def test_system_bytes_counts_the_cell_grid(param_dir):

@@ -197,3 +197,50 @@ def test_forced_rebuild_keeps_invariants(param_dir):
     assert abs(orc.observables().ered - e0) <= 1e-10 * abs(e0)
     orc.run(30000)
     assert abs(orc.observables().ered - e0) <= 1e-10 * abs(e0)
+
+
+def overlapping_start(chains=(8, 8), seed=22, depth=0.5):
+    """A generated system with chain 1 moved rigidly so that its first
+    C-alpha sits inside the hard core of chain 0's first C-alpha (at `depth`
+    core diameters) and approaches it; bonds inside each chain stay intact.
+    The reference's start-up stops at the pair's negative collision time
+    (events.f:59-73)."""
+    sysd = pdmd.generate(list(chains), density=0.01, tstar=0.2, seed=seed)
+    sv = sysd["sv"].reshape(-1, 6).copy()
+    a, b0, b1 = 0, 4 * chains[0], 4 * (chains[0] + chains[1])
+    sigma = 3.3 / sysd["boxl"]  # C-alpha core diameter (protein.data), box units
+    shift = sv[a, :3] - sv[b0, :3] + np.array([depth * sigma, 0.0, 0.0])
+    sv[b0:b1, :3] += shift
+    sv[:, :3] -= np.round(sv[:, :3])
+    speed = 10.0 * np.abs(sv[:, 3:]).max()
+    sv[b0:b1, 3] -= speed  # chain 1 moves towards -x, onto bead a
+    out = dict(sysd)
+    out["sv"] = sv.ravel()
+    return out
+
+
+def test_overlapping_start_stops_at_negative_time(param_dir):
+    # the oracle's start-up reports the negative time (-12), not the
+    # negative calendar bin that follows from it; the host half of
+    # pdmd_system_add (no device call) accepts the same state, so the device
+    # start-up meets the same negative time
+    ovl = overlapping_start()
+    with pytest.raises(RuntimeError, match=r"\(-12\)"):
+        Oracle(ovl, seed=1, param_dir=param_dir)
+    d, keep = pdmd.Engine.make_desc(ovl, seed=1, canon=True)
+    assert pdmd.lib().pdmd_system_bytes(d) > 0, pdmd.lib().pdmd_last_error()
+    Oracle(pdmd.generate([8, 8], density=0.01, tstar=0.2, seed=22), seed=1, param_dir=param_dir)
+
+
+def test_restore_reports_exhausted_override_slots():
+    """A start in which several N-C pairs sit inside their wells around the
+    same beads (chains pushed into each other) needs more shoulder overrides
+    per bead than the 4 slots the device keeps.  The reference's N x N
+    ev_code matrix has no such limit, but its dynamics never produce such a
+    state: a non-terminal N-C pair stays inside its well only while the two
+    are H-bonded to each other (main.F90:1487-1560 bounces the others), so a
+    bead holds at most 3 overrides.  pdmd_system_add reports the overflow
+    from its host half, before any device call."""
+    d, keep = pdmd.Engine.make_desc(overlapping_start(seed=21), seed=1, canon=True)
+    assert pdmd.lib().pdmd_system_bytes(d) < 0
+    assert "override slots exhausted" in pdmd.lib().pdmd_last_error().decode()
