@@ -762,8 +762,11 @@ struct Lds {
     int o_blk, o_sup, o_wm, o_wx, o_blist, o_pair, o_sc, o_leaf, o_wd, o_wt, o_wy, o_slot, o_spec, o_down;
     int o_msc, o_batch;
     Leaf* gleaf;  // calendar leaves in HBM when they do not fit in LDS
+    // the round's footprint marks: a flat pointer into LDS (behind the
+    // leaves, when both fit there) or to SysDev::mark
+    unsigned* mark;
     int nblk, nsup, maxw, nw;
-    bool leaf_lds;
+    bool leaf_lds, mark_lds;
     __device__ Key* blk() const { return reinterpret_cast<Key*>(smem + o_blk); }
     __device__ Key* sup() const { return reinterpret_cast<Key*>(smem + o_sup); }
     __device__ int* wm() const { return reinterpret_cast<int*>(smem + o_wm); }  // work list: partner / bead
@@ -819,6 +822,8 @@ __device__ inline Lds lds_regs() {
     r.maxw = __builtin_amdgcn_readfirstlane(m.maxw);
     r.nw = __builtin_amdgcn_readfirstlane(m.nw);
     r.leaf_lds = __builtin_amdgcn_readfirstlane(m.leaf_lds ? 1 : 0) != 0;
+    r.mark = sgpr_ptr(m.mark);
+    r.mark_lds = __builtin_amdgcn_readfirstlane(m.mark_lds ? 1 : 0) != 0;
     return r;
 }
 __device__ inline Leaf leaf_get(const Lds& L, int k) { return L.leaf_lds ? L.lleaf()[k] : gld(L.gleaf + k); }
@@ -953,14 +958,16 @@ __device__ inline int shoulder_bead(const SysDev&, const Pair& P, int lane) {
     return (v >= 0 && v < S.N) ? v : -1;
 }
 // the marks live in HBM and are read and written at the device's L2
-__device__ inline void mark_or(unsigned* p, unsigned bits) {
-    __hip_atomic_fetch_or((GLB unsigned*)p, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Footprint marks of bead b through Lds::mark (flat: LDS when the system's
+// leaves and marks fit there, else SysDev::mark in HBM).
+__device__ inline void mark_or(const Lds& L, const SysDev&, int b, unsigned bits) {
+    __hip_atomic_fetch_or(L.mark + b, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline unsigned mark_get(const unsigned* p) {
-    return __hip_atomic_load((GLB unsigned*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline unsigned mark_get(const Lds& L, const SysDev&, int b) {
+    return __hip_atomic_load(L.mark + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void mark_clear(unsigned* p) {
-    __hip_atomic_store((GLB unsigned*)p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void mark_clear(const Lds& L, const SysDev&, int b) {
+    __hip_atomic_store(L.mark + b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // A wavefront's place in a multi-wavefront round: -1 alone; 0 the round's
 // first event (always committed, writes as it goes); k >= 1 a speculative
@@ -980,12 +987,12 @@ __device__ inline int spec_abort(const Lds& L, int bk, int& nb) {
     return -1;
 }
 
-__device__ inline void round_clear_marks(const SysDev& S, int dent, int i, int j, int sp) {
+__device__ inline void round_clear_marks(const Lds& L, const SysDev& S, int dent, int i, int j, int sp) {
     const int lane = lane_id();
-    if (dent >= 0) mark_clear(S.mark + dent);
-    if (lane == 0) mark_clear(S.mark + i);
-    if (lane == 1 && j >= 0) mark_clear(S.mark + j);
-    if (sp >= 0) mark_clear(S.mark + sp);
+    if (dent >= 0) mark_clear(L, S, dent);
+    if (lane == 0) mark_clear(L, S, i);
+    if (lane == 1 && j >= 0) mark_clear(L, S, j);
+    if (sp >= 0) mark_clear(L, S, sp);
 }
 // Events of a round committed: the prefix before the first cut-off event,
 // before the first event after an error, and before the first event whose
@@ -1674,12 +1681,12 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         const unsigned sbit = 1u << bk, lbit = 1u << (8 + bk), fbit = 1u << (16 + bk);
         sp = shoulder_bead(S, P, lane);
         if (lane == 0) B.wdone[bk] = 0;
-        if (dent >= 0) mark_or(S.mark + dent, lbit | (mfull ? fbit : 0u));
-        if (lane == 0) mark_or(S.mark + i, sbit | lbit | fbit);
-        if (lane == 1 && j >= 0) mark_or(S.mark + j, sbit | lbit | fbit);
+        if (dent >= 0) mark_or(L, S, dent, lbit | (mfull ? fbit : 0u));
+        if (lane == 0) mark_or(L, S, i, sbit | lbit | fbit);
+        if (lane == 1 && j >= 0) mark_or(L, S, j, sbit | lbit | fbit);
         sp_w = (P.flags & kPairWroteSp) != 0;
         const bool sp_r = (P.flags & kPairReadsSp) != 0;
-        if (bk == 0 && sp_w && sp >= 0) mark_or(S.mark + sp, sbit);
+        if (bk == 0 && sp_w && sp >= 0) mark_or(L, S, sp, sbit);
         bar_to(nb, 1);
         PH_END(S, kPhBMark, tph);
         if (bk >= 1) {
@@ -1689,11 +1696,11 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                 const int q = base + lane;
                 if (q < npairs) {
                     const int m = (base == 0 && q < ncand ? cent : v_wm[q]) & 0xFFFFFF;
-                    if (mark_get(S.mark + m) & lowS) bad = true;
+                    if (mark_get(L, S, m) & lowS) bad = true;
                 }
             }
             if (dent >= 0) {
-                const unsigned mk = mark_get(S.mark + dent);
+                const unsigned mk = mark_get(L, S, dent);
                 if (mk & lowS) {
                     bad = true;
                 } else if ((mk & lowL) && first) {
@@ -1706,9 +1713,9 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                         bad = true;
                 }
             }
-            if (lane == 0 && (mark_get(S.mark + i) & (lowS | lowL))) bad = true;
-            if (lane == 1 && j >= 0 && (mark_get(S.mark + j) & (lowS | lowL))) bad = true;
-            if (sp_r && sp >= 0 && (mark_get(S.mark + sp) & lowS)) bad = true;
+            if (lane == 0 && (mark_get(L, S, i) & (lowS | lowL))) bad = true;
+            if (lane == 1 && j >= 0 && (mark_get(L, S, j) & (lowS | lowL))) bad = true;
+            if (sp_r && sp >= 0 && (mark_get(L, S, sp) & lowS)) bad = true;
             // a down bead whose full recompute is not in the batch would
             // read rows that were not checked
             if (mfull && !(spec && v_spec[myslot].off >= 0)) bad = true;
@@ -1719,7 +1726,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         // (the others learn theirs at the next barrier)
         if (bk >= B.cut) {
             bar_to(nb, kRoundBars);
-            round_clear_marks(S, dent, i, j, bk == 0 && sp_w ? sp : -1);
+            round_clear_marks(L, S, dent, i, j, bk == 0 && sp_w ? sp : -1);
             return;
         }
     }
@@ -1989,7 +1996,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         bar_to(nb, kRoundBars);
         PH_END(S, kPhBKeys, tph);
         // every wavefront has checked: clear this one's marks
-        round_clear_marks(S, dent, i, j, bk == 0 && sp_w ? sp : -1);
+        round_clear_marks(L, S, dent, i, j, bk == 0 && sp_w ? sp : -1);
         if (bk >= batch_accepted(B)) return;
         if (bk >= 1 && lane == 0) {
             gst(S.rec + (i), P.ri);
@@ -3260,6 +3267,10 @@ __device__ Lds make_lds(const SysDev&, int maxw, int lds_bytes, int nw, int wave
     L.o_leaf = p;
     L.leaf_lds = N3 <= (lds_bytes - p) / (int)sizeof(Leaf);
     L.gleaf = S.leaf;
+    // footprint marks behind the leaves, when both fit
+    const int o_mark = (p + N3 * (int)sizeof(Leaf) + 15) / 16 * 16;
+    L.mark_lds = L.leaf_lds && o_mark + 4 * S.N <= lds_bytes;
+    L.mark = L.mark_lds ? reinterpret_cast<unsigned*>(smem + o_mark) : S.mark;
     return L;
 }
 
@@ -3565,6 +3576,8 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
     copy_tables(S.tab, const_cast<DevTables*>(&tabs()));
     if (L.leaf_lds)
         for (int e = (int)threadIdx.x; e < N3; e += (int)blockDim.x) L.lleaf()[e] = gld(S.leaf + (e));
+    if (L.mark_lds)
+        for (int b = (int)threadIdx.x; b < N; b += (int)blockDim.x) L.mark[b] = 0u;
     __syncthreads();
     const DevTables& T = tabs();
     Sc& c = *L.sc();  // the master scalars on wavefront 0
