@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--capacity-nsys", type=int, default=256,
+                    help="systems of the same distribution in the capacity run (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="arrival window")
     ap.add_argument("--serve-min", type=float, default=1e4, help="collisions per request, log-uniform lower end")
@@ -315,6 +317,32 @@ def main():
         pdt = time.perf_counter() - t0
         plain = pps * len(ids) / pdt
 
+    # capacity: more systems of the same distribution resident at once (one
+    # workgroup each, up to one per CU), same per-step collisions and W;
+    # reported beside the headline, which stays the 64-system workload
+    capacity = None
+    if args.capacity_nsys > 0:
+        cspecs = uneven64(nsys=args.capacity_nsys)
+        csys = build_systems(cspecs)
+        ceng = pdmd.Engine()
+        try:
+            cids = [ceng.add_system(s, seed=sp["seed"] + 17) for s, sp in zip(csys, cspecs)]
+            ceng.set_waves(waves)
+            ceng.advance(cids, [args.per_step] * len(cids))
+            csteps = 2
+            t0 = time.perf_counter()
+            for _ in range(csteps):
+                ceng.advance(cids, [args.per_step] * len(cids))
+            cdt = time.perf_counter() - t0
+            capacity = {
+                "systems": len(cids), "beads_total": int(sum(ceng.nbeads[i] for i in cids)),
+                "collisions_per_s": round(args.per_step * len(cids) * csteps / cdt, 1),
+                "ms_per_step": round(cdt / csteps * 1e3, 3), "waves_per_system": waves,
+                "note": "systems 0-63 are the headline workload; the rest are further draws of its distribution",
+            }
+        finally:
+            ceng.close()
+
     serve = None
     if not args.no_serve:
         serve = serve_run(systems, seeds, args, waves)
@@ -377,6 +405,7 @@ def main():
         "p50_latency_s": None if serve is None else serve["p50_latency_s"],
         "p99_latency_s": None if serve is None else serve["p99_latency_s"],
         "serving": serve,
+        "capacity": capacity,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 3),
