@@ -3487,20 +3487,6 @@ __device__ int select_batch(const Lds&, Sc& c, int N, int N3, int kmax) {
         }
         const int e = key_idx(k);
         if (K > 0 && e >= N) break;
-        // a pair entry that shares a bead with an earlier popped event reads
-        // a state that event rewrites, so the round's footprint check would
-        // cut it for certain: the round ends before it instead (popping on
-        // would only add events behind that cut)
-        if (K > 0) {
-            const int pe = key_p(k);
-            bool clash = false;
-            for (int m = 0; m < K; ++m) {
-                const unsigned q = (unsigned)__builtin_amdgcn_readlane((int)pip, m);
-                const int mi = (int)(q >> 16), mp = (int)(q & 0xFFFFu) - 2;
-                clash |= e == mi || e == mp || (pe >= 0 && (pe == mi || pe == mp));
-            }
-            if (clash) break;
-        }
         if (lane == 0) B.key[K] = k;
         if (lane == K) pip = k.ip;
         ++K;
