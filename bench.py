@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 6144-bead plain system and the 11-system temperature ladder")
     ap.add_argument("--capacity-nsys", type=int, default=256,
                     help="systems of the same distribution in the capacity run (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson request arrivals per second")
@@ -317,6 +319,45 @@ def main():
         pdt = time.perf_counter() - t0
         plain = pps * len(ids) / pdt
 
+    # the other GPU configurations at their shapes: one 6144-bead system
+    # (96 x 16 residues, rho* = 0.01, T* = 0.20) alone, with its list
+    # rebuilds timed by its own device clock, and the 11-copy temperature
+    # ladder of it in one launch
+    configs = None
+    if not args.no_configs:
+        configs = {}
+        ladder = [0.18, 0.20, 0.22, 0.24, 0.26, 0.28, 0.30, 0.35, 0.40, 0.45, 0.50]
+        for name, temps in (("plain_6144", [0.20]), ("ladder_11x6144", ladder)):
+            geng = pdmd.Engine(waves=waves)
+            try:
+                gids = [geng.add_system(pdmd.generate([16] * 96, density=0.01, tstar=t, seed=6144 + q), seed=61 + q)
+                        for q, t in enumerate(temps)]
+                geng.advance(gids, [args.per_step] * len(gids))
+                g0 = [geng.stats(i) for i in gids]
+                t0 = time.perf_counter()
+                geng.advance(gids, [2 * args.per_step] * len(gids))
+                gdt = time.perf_counter() - t0
+                g1 = [geng.stats(i) for i in gids]
+                rec = {"systems": len(gids), "beads_each": 6144,
+                       "collisions_per_s": round(2 * args.per_step * len(gids) / gdt, 1)}
+                if name == "plain_6144":
+                    st0, st1 = g0[0], g1[0]
+                    nreb = st1.rebuilds - st0.rebuilds
+                    if nreb > 0:
+                        # lists (16 B per bead, 4 B per entry, 4 B per search
+                        # cell) and events() (53 B per up-pair, 16 B per leaf)
+                        n = st1.nbeads
+                        cells = (st1.cells_axis // st1.cell_factor) ** 3
+                        b = 16 * n + 4 * n * (st1.avg_up + st1.avg_dn) + 4 * cells + 53 * n * st1.avg_up + 16 * n
+                        per = (st1.rebuild_ms - st0.rebuild_ms) / nreb
+                        rec.update(list_rebuilds=int(nreb), list_rebuild_ms=round(per, 3),
+                                   list_rebuild_algorithmic_gbps=round(b / (per / 1e3) / 1e9, 2),
+                                   list_rebuild_share=round((st1.rebuild_ms - st0.rebuild_ms) /
+                                                            (st1.device_ms - st0.device_ms), 3))
+                configs[name] = rec
+            finally:
+                geng.close()
+
     # capacity: more systems of the same distribution resident at once (one
     # workgroup each, up to one per CU), same per-step collisions and W;
     # reported beside the headline, which stays the 64-system workload
@@ -406,6 +447,7 @@ def main():
         "p99_latency_s": None if serve is None else serve["p99_latency_s"],
         "serving": serve,
         "capacity": capacity,
+        "configs": configs,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 3),

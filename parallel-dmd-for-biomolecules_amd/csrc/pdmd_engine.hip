@@ -2697,16 +2697,20 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
     // per lane, its bead's non-empty neighbour cell ranges, [t][lane] in the
     // wavefront's own work area; wavefronts take 64-bead chunks in turn
     int2* rng = reinterpret_cast<int2*>(smem + L.o_wm);
+    // beads are taken in cell order (64 consecutive sorted places per
+    // chunk): the lanes of a chunk sit in the same or adjacent cells, so
+    // their candidate reads fall on shared cache lines
     for (int i0 = wave * kW; i0 < N; i0 += nw * kW) {
-        const int i = i0 + lane;
-        const bool act = i < N;
-        const int ii = act ? i : 0;
-        const int fi = gl1(S.cfine + (ii));
+        const int ps = i0 + lane;
+        const bool act = ps < N;
+        const int4 si = act ? gld(S.sinfo + (ps)) : make_int4(-1, 0, 0, 0);
+        const double4 bi = act ? gld(S.spos + (ps)) : make_double4(0.0, 0.0, 0.0, 0.0);
+        const int i = si.x;  // -1 on an idle lane
+        const int fi = si.z;
         const int ix = fi & 1023, iy = (fi >> 10) & 1023, iz = fi >> 20;
         // the coarse cell, as the binning step derived it from the fine one
         const int qx = min(ix / cq, Mq - 1), qy = min(iy / cq, Mq - 1), qz = min(iz / cq, Mq - 1);
-        const Rec bi = gld(S.rec + (ii));
-        const int4 oi = gld(S.ovr + (ii));
+        const int4 oi = gld(S.ovr + (act ? i : 0));
         int nr = 0;
 #pragma unroll
         for (int t = 0; t < 27; ++t) {
@@ -2772,7 +2776,7 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
                 dy += dy < 0 ? M : 0;
                 dz += dz < 0 ? M : 0;
                 if ((dx > 2 && dx < M - 2) || (dy > 2 && dy < M - 2) || (dz > 2 && dz < M - 2)) continue;
-                const int stat = static_t(inf[u].y, bi.topo);
+                const int stat = static_t(inf[u].y, si.y);
                 // ev_code(j,i), nbor.f:63: j's overrides exist only towards i's
                 int code = stat;
                 if ((oi.x >= 0 && (oi.x >> 8) == jb) || (oi.y >= 0 && (oi.y >> 8) == jb) ||
