@@ -1,11 +1,12 @@
-"""Standalone dispatches of the list-rebuild and predictor kernels on the
-uneven64 workload, for rocprofv3 trace and counter runs.  Prints one JSON
-line with the algorithmic bytes and FP64 operations of the dispatches it
-made, so that a profile's kernel durations and HBM counters give achieved
-GB/s and the FP64 fraction per kernel.
+"""Standalone dispatches of the list-rebuild, predictor and event-loop
+kernels, for rocprofv3 trace and counter runs.  Prints one JSON line; for
+the uneven64 list and predictor dispatches it carries their algorithmic
+bytes and FP64 operations, so that a profile's kernel durations and HBM
+counters give achieved GB/s and the FP64 fraction per kernel.
 
     python parallel-dmd-for-biomolecules_amd/kernel_report.py rebuild [--reps 5]
     python parallel-dmd-for-biomolecules_amd/kernel_report.py predict [--reps 5]
+    python parallel-dmd-for-biomolecules_amd/kernel_report.py run --chains 4 --residues 8 [--entries 30000]
 
 rebuild: the 64 systems, each advanced by --advance committed entries, then
 --reps forced list updates of all of them at once (pdmd_rebuild: one
@@ -15,6 +16,9 @@ lists; one workgroup per system in each).
 predict: every listed up-pair of every system through the predictor hook
 (pdmd_predict_pairs: one k_predict dispatch per system, one lane per pair),
 --reps times.
+run: one seeded system (--chains x --residues, --density, --tstar) alone,
+--advance entries, then one k_run dispatch of --entries entries (the last
+k_run dispatch of the process is the measured one).
 
 Algorithmic bytes (the units the bench's roofline uses):
   lists   16 B per bead read (FP32 position + identity) + 4 B per list
@@ -46,11 +50,31 @@ FLOPS_PER_PREDICTION = 37.0
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("kernel", choices=("rebuild", "predict"))
+    ap.add_argument("kernel", choices=("rebuild", "predict", "run"))
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--advance", type=int, default=2000, help="committed entries per system before the dispatches")
     ap.add_argument("--waves", type=int, default=4)
+    ap.add_argument("--chains", type=int, default=4)
+    ap.add_argument("--residues", type=int, default=8)
+    ap.add_argument("--density", type=float, default=0.0101)
+    ap.add_argument("--tstar", type=float, default=0.27)
+    ap.add_argument("--entries", type=int, default=30000)
     args = ap.parse_args()
+    if args.kernel == "run":
+        eng = pdmd.Engine(waves=args.waves)
+        try:
+            sid = eng.add_system(pdmd.generate([args.residues] * args.chains, args.density, args.tstar, seed=7),
+                                 seed=100)
+            eng.advance([sid], [args.advance])
+            eng.advance([sid], [args.entries])
+            st = eng.stats(sid)
+            if st.status:
+                raise SystemExit(f"device status {st.status}")
+            print(json.dumps({"kernel": "run", "beads": st.nbeads, "waves": args.waves, "entries": args.entries,
+                              "launch_ms": round(eng.last_launch_ms(), 3)}), flush=True)
+        finally:
+            eng.close()
+        return
     specs = uneven64()
     systems = build_systems(specs)
     eng = pdmd.Engine(waves=args.waves)
