@@ -220,8 +220,7 @@ int pdmd_log_read(pdmd_ctx* ctx, int32_t sys_id, int32_t max, pdmd_event_t* out,
 /* Test hook: the neighbour lists as the last rebuild left them (nbor.f's
  * nb / na_npt and dnnab / nnabdn, header.f:21): n_up[k] and n_dn[k] the
  * row lengths of bead k+1, up[k*cap + m] / dn[k*cap + m] its 1-based
- * partners (rows in the device's order: by coarse cell then index, or by
- * index up to 256 beads; the membership is the reference's); cap as
+ * partners (rows in nbor.f's order, as the reference builds them); cap as
  * pdmd_list_capacity returns.  Any pointer may be NULL. */
 int pdmd_list_capacity(pdmd_ctx* ctx, int32_t sys_id);
 int pdmd_lists(pdmd_ctx* ctx, int32_t sys_id, int32_t* n_up, int32_t* up, int32_t* n_dn, int32_t* dn);

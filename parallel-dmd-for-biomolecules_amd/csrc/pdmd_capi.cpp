@@ -67,6 +67,7 @@ struct HostSys {
     DevBuf<double2> bw;
     DevBuf<int> nb_up, nup, nb_dn, ndn, cfine, ckey, cstart, crun, csorted;
     DevBuf<int4> extra, ovr;
+    DevBuf<int2> cfp;
     DevBuf<Leaf> leaf;
     DevBuf<double4> old_r, spos;
     DevBuf<int4> sinfo;
@@ -360,6 +361,7 @@ int prepare_system(const Params& raw, const pdmd_system_desc* d, Prep& out) {
     std::memcpy(tab.rlsq, H.cut.rlsq, sizeof tab.rlsq);
     for (int k = 0; k < 5; ++k) tab.shder[k] = P.shder_dist[k];
     tab.epsilon1 = P.epsilon[1];
+    half_stencil(tab.halfst);
     // per-bead topology/identity words and own side-chain bond windows
     // (inputinfo.f:300-349, bond.f:21-31: Calpha owns code 10, N code 11, C code 12)
     std::vector<double2>& bw = out.bw;
@@ -403,7 +405,7 @@ int64_t device_bytes(const Prep& pp) {
     const int64_t sizes[] = {(int64_t)sizeof(DevTables), N * (int64_t)sizeof(Rec), N * 16 /*bw*/, N * 16 /*extra*/,
                              N * 16 /*ovr*/, N * 32 /*old_r*/, N * cap * 4 /*nb_up*/, N * cap * 4 /*nb_dn*/,
                              N * 4 /*nup*/, N * 4 /*ndn*/, (N + 3) * (int64_t)sizeof(Leaf), N * 4 /*cfine*/,
-                             N * 4 /*ckey*/, N * 4 /*csorted*/, N * 32 /*spos*/, N * 16 /*sinfo*/, N * 4 /*mark*/,
+                             N * 4 /*ckey*/, N * 8 /*cfp*/, N * 4 /*csorted*/, N * 32 /*spos*/, N * 16 /*sinfo*/, N * 4 /*mark*/,
                              (nc + 1) * 4 /*cstart*/, nc * 4 /*crun*/, (N + 3) * 8 /*stamp*/,
                              (int64_t)sizeof(SysState)};
     int64_t b = 0;
@@ -569,6 +571,7 @@ int prepared_add(pdmd_ctx* ctx, PreparedSystem* ps, int32_t* sys_id) {
     HIPCHK(H.leaf.alloc(N + 3));
     HIPCHK(H.cfine.alloc(N));
     HIPCHK(H.ckey.alloc(N));
+    HIPCHK(H.cfp.alloc(N));
     HIPCHK(H.csorted.alloc(N));
     HIPCHK(H.spos.alloc(N));
     HIPCHK(H.sinfo.alloc(N));
@@ -613,6 +616,7 @@ int prepared_add(pdmd_ctx* ctx, PreparedSystem* ps, int32_t* sys_id) {
     D.leaf = H.leaf.p;
     D.cfine = H.cfine.p;
     D.ckey = H.ckey.p;
+    D.cfp = H.cfp.p;
     D.cstart = H.cstart.p;
     D.crun = H.crun.p;
     D.csorted = H.csorted.p;

@@ -71,6 +71,10 @@ struct DevTables {
     double rlsq[51];
     double shder[5];
     double epsilon1;
+    // nbor.f's scan order of neighbour cells (cell_link.f:13-94): for the
+    // offset (dx, dy, dz) in [-2, 2]^3 at (dx+2) + 5 (dy+2) + 25 (dz+2), its
+    // place 1..62 in the half stencil, 0 where the opposite offset is in it
+    signed char halfst[128];
 };
 
 // Slots of SysState::phase (pdmd_stats_t::phase_cycles).
@@ -119,6 +123,7 @@ struct SysDev {
     Leaf* leaf;
     int* cfine;    // packed fine cell coordinates per bead
     int* ckey;     // coarse cell per bead
+    int2* cfp;     // [N] fine cell and the place in its list as cell_add.f builds it (1 = highest index)
     int* cstart;   // [ncoarse+1]
     int* crun;     // [ncoarse]
     int* csorted;  // [N]

@@ -131,6 +131,13 @@ template <class T>
 __device__ inline void gs1(T* p, T v) {
     *(GLB T*)p = v;
 }
+__device__ inline int2 gl2(const int2* p) {
+    const unsigned long long v = gl1(reinterpret_cast<const unsigned long long*>(p));
+    return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+}
+__device__ inline void gs2(int2* p, int2 v) {
+    gs1(reinterpret_cast<unsigned long long*>(p), (unsigned long long)(unsigned)v.x | ((unsigned long long)(unsigned)v.y << 32));
+}
 
 // The workgroup's dynamic LDS.  Views into it are kept as offsets and
 // turned into pointers from this array where they are used, so the compiler
@@ -245,6 +252,33 @@ __device__ inline int static_t(int ta, int tb) {
     const int r = static_t0(ta, tb);
     return (r == 15 && topo_nohbs(ta)) ? 1 : r;
 }
+// Where nbor.f finds the pair {a, b}, as one 64-bit key: the cell it scans
+// (cell_add.f's linear index, cells in ascending order), the place of that
+// cell's own bead in its list, the other cell's place in the scan
+// (cell_link.f's half stencil; 0 for the own cell), the other bead's place.
+// A bead's neighbour rows in nbor.f's order are its rows sorted by this key.
+// a and b: the beads' fine cells (10 bits per axis) and list places.
+__device__ inline unsigned long long nbor_key(int M, const DevTables& T, int2 a, int2 b) {
+    const int fa = a.x, fb = b.x, pa = a.y, pb = b.y, nc = M + 4;
+    int d[3];
+    for (int q = 0; q < 3; ++q) {
+        // periodic offset in [-(M / 2), M - M / 2), in [-2, 2] for neighbours
+        int t = ((fb >> (10 * q)) & 1023) - ((fa >> (10 * q)) & 1023);
+        t += t < 0 ? M : 0;
+        d[q] = t < M - M / 2 ? t : t - M;
+    }
+    auto lin = [&](int f) {
+        return 1ull + (unsigned long long)((f & 1023) + 2) + (unsigned long long)(((f >> 10) & 1023) + 2) * nc +
+               (unsigned long long)((f >> 20) + 2) * nc * nc;
+    };
+    if (d[0] == 0 && d[1] == 0 && d[2] == 0)
+        return (lin(fa) << 26) | ((unsigned long long)min(pa, pb) << 16) | (unsigned long long)max(pa, pb);
+    const int h = T.halfst[(d[0] + 2) + 5 * (d[1] + 2) + 25 * (d[2] + 2)];
+    if (h > 0) return (lin(fa) << 26) | ((unsigned long long)pa << 16) | ((unsigned long long)h << 10) | (unsigned)pb;
+    const int g = T.halfst[(2 - d[0]) + 5 * (2 - d[1]) + 25 * (2 - d[2])];
+    return (lin(fb) << 26) | ((unsigned long long)pb << 16) | ((unsigned long long)g << 10) | (unsigned)pa;
+}
+
 // ev_code(row, col) with row's override slots o
 __device__ inline int dyn_code(int4 o, int col, int stat) {
     if (o.x >= 0 && (o.x >> 8) == col) return o.x & 0xff;
@@ -2691,6 +2725,17 @@ __device__ void rebuild_lists(const SysDev&, const Lds&, Sc& c) {
         gst(S.sinfo + (p2), make_int4(jb, b.topo, gl1(S.cfine + jb), 0));
     }
     wg_sync(nw);
+    // each bead's place in its fine cell's list as cell_add.f builds it
+    // (head first, the highest index at 1): the beads of a fine cell sit
+    // in its coarse cell's sorted run in index order
+    for (int p2 = tid; p2 < N; p2 += nthr) {
+        const int4 si = gld(S.sinfo + (p2));
+        const int end = gl1(S.cstart + (gl1(S.ckey + si.x) + 1));
+        int pos = 1;
+#pragma unroll 4
+        for (int q2 = p2 + 1; q2 < end; ++q2) pos += gl1(&S.sinfo[q2].z) == si.z ? 1 : 0;
+        gs2(S.cfp + (si.x), make_int2(si.z, pos));
+    }
     PH_END(S, kPhLCopy, trl);
     int mx = 0, status = 0;
     const int noff = Mq > 2 ? 3 : Mq;  // distinct coarse offsets {0, +1, -1}
@@ -2875,6 +2920,23 @@ __device__ void rebuild_lists_dense(const SysDev&, const Lds&, Sc& c) {
     }
     for (int i = wave + nw * lane; i < N; i += nw * kW) cnt[i] = make_int2(0, 0);
     wg_sync(nw);
+    // fine cells and each bead's place in its fine cell's list (cell_add.f:
+    // head first, the highest index at 1), for the rows' nbor.f order: lane
+    // k counts the later beads of its cell, 64 candidates per LDS read
+    for (int k0 = wave * kW; k0 < N; k0 += nw * kW) {
+        const int k = k0 + lane;
+        const int f = k < N ? sfine[k] : -1;
+        int pos = 1;
+        for (int q0 = k0; q0 < N; q0 += kW) {
+            const int v = q0 + lane < N ? sfine[q0 + lane] : -2;
+#pragma unroll
+            for (int m = 0; m < kW; ++m) pos += (__builtin_amdgcn_readlane(v, m) == f && q0 + m > k) ? 1 : 0;
+        }
+        if (k < N) {
+            gs1(S.cfine + (k), f);
+            gs2(S.cfp + (k), make_int2(f, pos));
+        }
+    }
     PH_END(S, kPhLCells, trl);
     const unsigned long long below = (1ull << lane) - 1ull;
     int qh = 0, qn = 0;  // queue head and length (wavefront-uniform)
@@ -3042,6 +3104,95 @@ __device__ void events_all(const SysDev&, const DevTables&, const Lds&, Sc&) {
     rescan_all(L, N + 3);
 }
 
+// Rows in nbor.f's order: every bead's up and down rows sorted by
+// nbor_key, so that the lowest list position among partners tied in time
+// is the one eventredo_up.f's strict < takes in nbor.f's row, and down
+// rows are walked in partial_events.f's order.  One lane per row (row r:
+// bead r / 2, its up row for even r, its down row for odd r): the row's
+// entries and keys in the wavefront's work area (kSortMax each per lane,
+// [slot][lane]), the row in up to five 16-byte loads and all its partner
+// loads in flight, then every entry written at its rank; a longer row is
+// sorted by the whole wavefront, one lane per entry.
+constexpr int kSortMax = 20;
+static_assert(12 * kSortMax * kW <= 20 * kMinMaxw, "row sort staging exceeds a wavefront's work area");
+__device__ void sort_rows(const SysDev&, const DevTables&, const Lds&) {
+    const Lds& L = ldsv();
+    const SysDev& S = sdev();
+    const DevTables& T = tabs();
+    const int lane = lane_id(), wave = wave_id(), nw = L.nw, N = S.N, cap = S.cap, M = S.M;
+    unsigned long long* const keys = reinterpret_cast<unsigned long long*>(smem + L.o_wm);  // [kSortMax][kW]
+    int* const ents = reinterpret_cast<int*>(keys + kSortMax * kW);                        // [kSortMax][kW]
+    const bool vec = (cap & 3) == 0;  // rows of whole int4s (row bases are 16-byte aligned)
+    for (int r0 = wave * kW; r0 < 2 * N; r0 += nw * kW) {
+        const int r = r0 + lane;
+        const bool act = r < 2 * N;
+        const int x = act ? r >> 1 : 0;
+        int* const row = ((r & 1) == 0 ? S.nb_up : S.nb_dn) + (size_t)x * cap;
+        const int n = act ? min(gl1(((r & 1) == 0 ? S.nup : S.ndn) + (x)), cap) : 0;
+        const int2 cx = gl2(S.cfp + (x));
+        if (n >= 2 && n <= kSortMax) {
+            if (vec) {
+                // reads up to the next multiple of 4 (<= kSortMax, <= cap)
+                int4 v[kSortMax / 4];
+#pragma unroll
+                for (int u = 0; u < kSortMax / 4; ++u)
+                    if (4 * u < n) v[u] = gld(reinterpret_cast<const int4*>(row) + u);
+#pragma unroll
+                for (int u = 0; u < kSortMax / 4; ++u)
+                    if (4 * u < n) {
+                        ents[(4 * u) * kW + lane] = v[u].x;
+                        ents[(4 * u + 1) * kW + lane] = v[u].y;
+                        ents[(4 * u + 2) * kW + lane] = v[u].z;
+                        ents[(4 * u + 3) * kW + lane] = v[u].w;
+                    }
+            } else {
+                for (int k = 0; k < n; ++k) ents[k * kW + lane] = gl1(row + (k));
+            }
+            {
+                int e[kSortMax];
+                int2 cb[kSortMax];
+#pragma unroll
+                for (int u = 0; u < kSortMax; ++u) e[u] = u < n ? ents[u * kW + lane] & 0xFFFFFF : x;
+#pragma unroll
+                for (int u = 0; u < kSortMax; ++u)
+                    if (u < n) cb[u] = gl2(S.cfp + (e[u]));
+#pragma unroll
+                for (int u = 0; u < kSortMax; ++u)
+                    if (u < n) keys[u * kW + lane] = nbor_key(M, T, cx, cb[u]);
+            }
+            for (int k = 0; k < n; ++k) {
+                const unsigned long long kk = keys[k * kW + lane];
+                int rk = 0;
+#pragma unroll
+                for (int m = 0; m < kSortMax; ++m) rk += m < n && keys[m * kW + lane] < kk ? 1 : 0;
+                gs1(row + (rk), ents[k * kW + lane]);
+            }
+        }
+        // rows longer than the staging: the wavefront sorts each in turn
+        unsigned long long big = __ballot(n > kSortMax);
+        while (big) {
+            const int src = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const int rb_ = r0 + src, xb = rb_ >> 1, nbig = __shfl(n, src, kW);
+            int* const rb = ((rb_ & 1) == 0 ? S.nb_up : S.nb_dn) + (size_t)xb * cap;
+            const int2 cxb = gl2(S.cfp + (xb));
+            const int e = lane < nbig ? gl1(rb + (lane)) : 0;
+            const unsigned long long kk =
+                lane < nbig ? nbor_key(M, T, cxb, gl2(S.cfp + (e & 0xFFFFFF))) : ~0ull;
+            int rk = 0;
+            for (int m = 0; m < nbig; ++m) {
+                const unsigned long long km = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)kk, m) |
+                                              ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(kk >> 32), m) << 32);
+                rk += km < kk ? 1 : 0;
+            }
+            wave_sync_mem();  // every lane has read its entry before any is written
+            if (lane < nbig) gs1(rb + (rk), e);
+            wave_sync_mem();
+        }
+    }
+    wg_sync(nw);
+}
+
 // Wrap and lists; all wavefronts of the system take part.
 __device__ void wrap_and_lists(const SysDev&, const Lds&, Sc&) {
     const Lds& L = ldsv();
@@ -3065,6 +3216,7 @@ __device__ void wrap_and_lists(const SysDev&, const Lds&, Sc&) {
         rebuild_lists_dense(S, L, c);
     else
         rebuild_lists(S, L, c);
+    sort_rows(S, tabs(), L);
 }
 
 // Wrap, lists and full reschedule; all wavefronts of the system take part.
@@ -3584,7 +3736,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
         for (int b = (int)threadIdx.x; b < N; b += (int)blockDim.x) L.mark[b] = 0u;
     __syncthreads();
     const DevTables& T = tabs();
-    Sc& c = *L.sc();  // the master scalars on wavefront 0
+    Sc& c = *L.sc();  // wavefront 0: the master scalars; others: their own copy
     Batch& B = *L.batch();
     if (threadIdx.x == 0) {
         load_sc(S.st, c);
@@ -3673,11 +3825,15 @@ __global__ __launch_bounds__(64 * kMaxWaves) void k_run(const SysDev* __restrict
         // a round of K >= 2 pair events, one per wavefront; a speculative
         // wavefront's scalars are the master's as its event comes up
         if (wave >= 1 && wave < K && lane == 0) {
+            // the copy was taken before the master counted the round's first
+            // event: the event number comes from the master, so that every
+            // event of the round stamps its adds with its own number (equal
+            // stamps would order tied entries by index, not last-added first)
             Sc& ck = *L.sc();
             ck.old_tfalse = B.key[wave - 1].t;
             ck.tfalse = B.key[wave].t;
             ck.nbin = key_bin(B.key[wave]);
-            ck.coll = c.coll + wave;
+            ck.coll = L.msc()->coll + wave;
             ck.status = 0;
         }
         wave_sync();

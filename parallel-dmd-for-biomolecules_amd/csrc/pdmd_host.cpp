@@ -282,6 +282,28 @@ int static_code(const Topology& t, int a, int b) {
     return 1;
 }
 
+void half_stencil(signed char out[125]) {
+    for (int k = 0; k < 125; ++k) out[k] = 0;
+    // the offsets in cell_link.f's two loops, in its order: first the
+    // nearest shell (dz in 0, 1, -1; dy in 0, 1; dx in 0, 1, -1), then the
+    // second shell (dz in 0, 1, -1, 2, -2; dy in 0, 1, 2; dx in 2, -2 only
+    // where |dz| and dy stay within the first shell); with dy = 0 it keeps
+    // dx >= 1 for dz <= 0 and dx >= 0 for dz >= 1, so that each pair of
+    // cells is scanned from one side
+    static const int o[5] = {0, 1, -1, 2, -2}, oy[3] = {0, 1, 2};
+    int n = 0;
+    auto put = [&](int dx, int dy, int dz) {
+        if (dy == 0 && (dz < 1 ? dx < 1 : dx < 0)) return;
+        out[(dx + 2) + 5 * (dy + 2) + 25 * (dz + 2)] = (signed char)++n;
+    };
+    for (int iz = 0; iz < 3; ++iz)
+        for (int iy = 0; iy < 2; ++iy)
+            for (int ix = 0; ix < 3; ++ix) put(o[ix], o[iy], o[iz]);
+    for (int iz = 0; iz < 5; ++iz)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int ix = (iz < 3 && iy != 2) ? 3 : 0; ix < 5; ++ix) put(o[ix], oy[iy], o[iz]);
+}
+
 void nbor_setup(const Params& p, const Topology& t, const std::vector<int>& ident, Cutoffs& c) {
     // nbor_setup.f:13-118.  Scan domain: the first chain of every species
     // (intra-chain pairs) and the first chains of every pair of species.

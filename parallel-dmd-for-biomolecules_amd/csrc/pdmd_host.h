@@ -56,6 +56,11 @@ struct Topology {
 // 0-based bead indices).  Closed form of make_code.f:72-563.
 int static_code(const Topology& t, int a, int b);
 
+// cell_link.f:13-94 (n_wrap = 2): the place 1..62 of each neighbour-cell
+// offset in the order nbor.f scans them, at (dx+2) + 5 (dy+2) + 25 (dz+2);
+// 0 for the offsets whose opposite is scanned instead, and for (0, 0, 0).
+void half_stencil(signed char out[125]);
+
 // nbor_setup.f:13-118: neighbour cutoffs per code, sig_max_all and hdelr.
 // Dynamic overrides present at set-up are ignored there as in the reference
 // (ev_code <= 26 filter), so only static codes matter.

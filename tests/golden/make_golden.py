@@ -252,7 +252,65 @@ def scattered(sysd, seed, gap=4.5):
     return out
 
 
-def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1, mirror=False, scatter=False):
+def reflected(sysd, seed, box=64.0, grid=2.0 ** -20, gap=4.5, nplane=16):
+    """A -Dchaptype=2 start symmetric under the reflection x -> -x (box
+    units; the planes x = 0 and x = 1/2 are its mirrors): the first chain
+    moved to x near 1/4 and the second chain its mirror image (same
+    sequence, vx negated); of the unbonded second-species beads, the first
+    `nplane` lie on the plane x = 0 with vx = 0 and the rest come in mirror
+    pairs.  Positions sit on a 2^-20 grid in a power-of-two box, so that a
+    plane bead predicts bit-identical collision times with the two beads of
+    a mirror pair: its full recomputes meet ties between two partners of one
+    row, which the reference breaks by its row order (nbor.f)."""
+    rng = np.random.default_rng(seed)
+    sv = np.array(sysd["sv"], np.float64).reshape(-1, 6).copy()
+    B0 = float(sysd["boxl"])
+    n1 = 4 * L1
+    # chain 1: contiguous, centred at x = 1/4, inside 0.08 < x < 0.42
+    rel = sv[:n1, :3] - sv[0, :3]
+    rel = rel - np.round(rel)
+    x = rel * B0 / box
+    x = x - x.mean(0) + np.array([0.25, 0.0, 0.0])
+    if x[:, 0].min() < 0.08 or x[:, 0].max() > 0.42:
+        raise SystemExit("chain too wide for the half box")
+    x = np.round(x / grid) * grid
+    sv[:n1, :3] = x
+    sv[n1:2 * n1, :3] = x * np.array([-1.0, 1.0, 1.0])
+    sv[n1:2 * n1, 3:] = sv[:n1, 3:] * np.array([-1.0, 1.0, 1.0])
+    placed = list(sv[:2 * n1, :3])
+
+    def far(p):
+        d = np.array(placed) - p
+        d = d - np.round(d)
+        return np.sqrt((d * d).sum(1)).min() * box >= gap
+
+    k = 2 * n1
+    N = sv.shape[0]
+    while k < N:
+        if k < 2 * n1 + nplane:
+            p = np.round(np.array([0.0, rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5)]) / grid) * grid
+            if far(p):
+                sv[k, :3] = p
+                sv[k, 3] = 0.0
+                placed.append(p)
+                k += 1
+        else:
+            p = np.round(np.array([rng.uniform(0.05, 0.45), rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5)]) / grid) * grid
+            q = p * np.array([-1.0, 1.0, 1.0])
+            if far(p) and far(q) and 2.0 * p[0] * box >= gap:
+                sv[k, :3] = p
+                sv[k + 1, :3] = q
+                sv[k + 1, 3:] = sv[k, 3:] * np.array([-1.0, 1.0, 1.0])
+                placed += [p, q]
+                k += 2
+    out = dict(sysd)
+    out["sv"] = sv.ravel()
+    out["boxl"] = box
+    return out
+
+
+def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1, mirror=False, scatter=False,
+             reflect=False):
     import pdmd
     rng = np.random.default_rng(seed)
     seq1 = [int(x) for x in rng.integers(9, 29, L1)]
@@ -263,6 +321,8 @@ def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=
         sysd = mirrored(sysd)
     if scatter:
         sysd = scattered(sysd, seed)
+    if reflect:
+        sysd = reflected(sysd, seed)
     work = os.path.join(OUT, "work_" + name)
     if os.path.isdir(work):
         shutil.rmtree(work)
@@ -304,6 +364,10 @@ def main():
     run_case("ref_2species_tie", exe, seed=31, tstar=0.20, density=0.015, nsteps=200000, mirror=True)
     # -Dchaptype=2: the second species (2 chains x 6 residues) unbonded
     run_case("ref_2species_ch2", exe_c2, seed=4245, tstar=0.20, density=0.01, nsteps=200000, scatter=True)
+    # partner ties: a start symmetric under x -> -x, so that beads on the
+    # mirror plane meet bit-identical times with two partners of one row
+    # and rows' nbor.f order decides
+    run_case("ref_2species_ptie", exe_c2, seed=50, tstar=0.20, density=0.01, nsteps=200000, reflect=True)
 
 
 if __name__ == "__main__":

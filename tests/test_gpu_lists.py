@@ -1,8 +1,8 @@
 """Neighbour lists on the device against the oracle's (nbor.f:23-137
 restated, pinned by the reference's own lists in test_oracle_reference.py):
 right after set-up every bead's up and down rows hold the oracle's partners
-(compared as sets: the device orders rows by cell, the reference by its
-cell traversal), the longest row and the totals agree; after 20000 entries
+in the same order (the device sorts each row into the order of nbor.f's
+cell traversal after its search), the longest row and the totals agree; after 20000 entries
 (several list rebuilds at the syncs) the rows agree again.  Systems of up
 to 256 beads build their lists with the all-pairs stencil filter, larger
 ones through the cell sort; both are covered, at 1 and 4 wavefronts."""
@@ -25,8 +25,8 @@ def _oracle_rows(orc, n):
 def _same_rows(eng, sid, orc, n):
     gup, gdn = eng.lists(sid)
     for i in range(1, n + 1):
-        assert np.array_equal(np.sort(gup[i - 1]), np.sort(orc.nb_up(i))), f"up row of bead {i}"
-        assert np.array_equal(np.sort(gdn[i - 1]), np.sort(orc.nb_dn(i))), f"down row of bead {i}"
+        assert np.array_equal(gup[i - 1], orc.nb_up(i)), f"up row of bead {i}"
+        assert np.array_equal(gdn[i - 1], orc.nb_dn(i)), f"down row of bead {i}"
 
 
 @pytest.mark.parametrize("chains,density,tstar", [
@@ -35,7 +35,9 @@ def _same_rows(eng, sid, orc, n):
     ([4] * 16, 0.03, 0.35),      # 256 beads, the largest of the stencil filter
     ([8] * 9, 0.012, 0.30),      # 288 beads, cell sort
     ([16] * 8, 0.01, 0.20),      # 512 beads, cell sort
-], ids=["128", "240", "256", "288", "512"])
+    ([8] * 4, 0.3, 0.30),        # 128 beads, dense: a down row longer than 20
+    ([8] * 9, 0.25, 0.30),       # 288 beads, dense, cell sort: three rows longer than 20
+], ids=["128", "240", "256", "288", "512", "128d", "288d"])
 @pytest.mark.parametrize("waves", [1, 4])
 def test_list_lengths_match_oracle(param_dir, chains, density, tstar, waves):
     sysd = pdmd.generate(chains, density=density, tstar=tstar, seed=77)

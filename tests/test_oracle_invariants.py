@@ -244,3 +244,69 @@ def test_restore_reports_exhausted_override_slots():
     d, keep = pdmd.Engine.make_desc(overlapping_start(seed=21), seed=1, canon=True)
     assert pdmd.lib().pdmd_system_bytes(d) < 0
     assert "override slots exhausted" in pdmd.lib().pdmd_last_error().decode()
+
+
+def nbor_half_stencil():
+    """cell_link.f:13-94 (n_wrap = 2): the order of the 62 neighbour-cell
+    offsets (dx, dy, dz) that nbor.f scans after a cell's own beads."""
+    o, oy = (0, 1, -1, 2, -2), (0, 1, 2)
+
+    def kept(dx, dy, dz):
+        return dy != 0 or (dx >= 1 if dz < 1 else dx >= 0)
+
+    order = [(dx, dy, dz) for dz in o[:3] for dy in o[:2] for dx in o[:3] if kept(dx, dy, dz)]
+    for iz, dz in enumerate(o):
+        for iy, dy in enumerate(oy):
+            lo = 3 if (iz < 3 and iy != 2) else 0
+            order += [(dx, dy, dz) for dx in o[lo:] if kept(dx, dy, dz)]
+    return {d: n + 1 for n, d in enumerate(order)}
+
+
+def nbor_order_key(cell, pos, M, H, a, b):
+    """Where nbor.f finds the pair {a, b}: (cell scanned, the scanned
+    cell's own bead's place in its linked list, stencil index (0 for the
+    own cell), the other bead's place).  cell[k]: linear cell of
+    cell_add.f; pos[k]: 1 + the beads of k's cell with a higher index."""
+    nc = M + 4
+    ca, cb = cell[a] - 1, cell[b] - 1
+    fa = (ca % nc, (ca // nc) % nc, ca // (nc * nc))
+    fb = (cb % nc, (cb // nc) % nc, cb // (nc * nc))
+    d = tuple(((fb[q] - fa[q]) % M + M // 2) % M - M // 2 for q in range(3))
+    if d == (0, 0, 0):
+        first, second = (a, b) if pos[a] < pos[b] else (b, a)
+        return (cell[a], pos[first], 0, pos[second])
+    if d in H:
+        return (cell[a], pos[a], H[d], pos[b])
+    return (cell[b], pos[b], H[(-d[0], -d[1], -d[2])], pos[a])
+
+
+@pytest.mark.parametrize("chains,seed,dens", [([16] * 8, 51, 0.01), ([24] * 6, 52, 0.02), ([8] * 30, 53, 0.005)])
+def test_row_order_key_reproduces_nbor_order(param_dir, chains, seed, dens):
+    """Every neighbour row of the oracle (nbor.f's order, pinned by the
+    reference's own lists) comes out of sorting the row's partners by
+    nbor_order_key: the tie-break among a row's partners on the device
+    follows it."""
+    from oracle import lib as orc_lib
+    H = nbor_half_stencil()
+    assert len(H) == 62 and all(max(abs(x) for x in d) <= 2 for d in H)
+    assert all((-d[0], -d[1], -d[2]) not in H for d in H)
+    sysd = pdmd.generate(chains, density=dens, tstar=0.25, seed=seed)
+    orc = Oracle(sysd, seed=3, param_dir=param_dir, canon=False)
+    N, M = orc.N, orc.stats().num_cell - 4
+    assert M >= 5
+    cell = {k: orc_lib().orc_cell_of(orc.h, k) for k in range(1, N + 1)}
+    pos = {}
+    members = {}
+    for k in range(1, N + 1):
+        members.setdefault(cell[k], []).append(k)
+    for c, ks in members.items():
+        for k in ks:
+            pos[k] = 1 + sum(1 for q in ks if q > k)
+    n_checked = 0
+    for x in range(1, N + 1):
+        for row in (orc.nb_up(x), orc.nb_dn(x)):
+            want = [int(y) for y in row]
+            got = sorted(want, key=lambda y: nbor_order_key(cell, pos, M, H, x, y))
+            assert got == want, (x, want, got)
+            n_checked += len(want) > 1
+    assert n_checked > N // 2
