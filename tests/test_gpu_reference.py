@@ -8,7 +8,7 @@ it is compared event by event and bit for bit with the device's log (the
 first 2e4 entries directly, the whole run through block digests), with the
 resolved-coltype tally, the capture-draw count and the final state --
 positions, velocities, calendar, identities, bptnr, extra_repuls -- at 1 and
-4 wavefronts per system; the device's checkover.f / check_nc_int.f kernel
+4 wavefronts per system (the two tie fixtures also at 2 and 8); the device's checkover.f / check_nc_int.f kernel
 against the reference's own messages on the final and two planted states;
 and the device's -Drunr restore (main.F90:240-321) of the final state
 against the reference's.
@@ -37,6 +37,21 @@ def _sysd(fx, sv):
 @pytest.mark.parametrize("waves", [1, 4])
 @pytest.mark.parametrize("name,flags", CASES, ids=[c[0] for c in CASES])
 def test_device_matches_reference_serial_loop(name, flags, waves):
+    _serial_loop(name, flags, waves)
+
+
+@pytest.mark.parametrize("waves", [2, 8])
+@pytest.mark.parametrize("name,flags", [c for c in CASES if c[0] in ("ref_2species_tie", "ref_2species_ptie")],
+                         ids=["ref_2species_tie", "ref_2species_ptie"])
+def test_device_tie_fixtures_at_other_widths(name, flags, waves):
+    """The planted calendar ties and partner ties with rounds of 2 and of
+    up to 8 events: every event of a round stamps its adds with its own
+    event number, so tied entries added by two events of one round keep the
+    reference's last-added-first order."""
+    _serial_loop(name, flags, waves)
+
+
+def _serial_loop(name, flags, waves):
     fx = _fixture(name)
     n = int(fx["loop_n"])
     print(f"reference loop {name}: {n} entries at W={waves}", flush=True)
