@@ -40,10 +40,26 @@ def _same_rows(eng, sid, orc, n):
 ], ids=["128", "240", "256", "288", "512", "128d", "288d"])
 @pytest.mark.parametrize("waves", [1, 4])
 def test_list_lengths_match_oracle(param_dir, chains, density, tstar, waves):
+    _lists_match(param_dir, chains, density, tstar, waves, 0)
+
+
+@pytest.mark.parametrize("chains,density,tstar,cap", [
+    ([8] * 4, 0.0101, 0.27, 17),  # dense path
+    ([8] * 4, 0.3, 0.30, 25),     # dense path, a row longer than 20
+    ([8] * 9, 0.25, 0.30, 27),    # cell sort, rows longer than 20
+    ([16] * 8, 0.01, 0.20, 18),   # cell sort
+], ids=["128-17", "128d-25", "288d-27", "512-18"])
+def test_rows_in_order_at_capacities_off_a_multiple_of_4(param_dir, chains, density, tstar, cap):
+    """Row capacities that are not a multiple of 4 (the sort then reads rows
+    entry by entry instead of 16 bytes at a time)."""
+    _lists_match(param_dir, chains, density, tstar, 4, cap)
+
+
+def _lists_match(param_dir, chains, density, tstar, waves, cap):
     sysd = pdmd.generate(chains, density=density, tstar=tstar, seed=77)
     eng = pdmd.Engine(waves=waves)
     try:
-        sid = eng.add_system(sysd, seed=5)
+        sid = eng.add_system(sysd, seed=5, list_capacity=cap)
         n = eng.nbeads[sid]
         orc = Oracle(sysd, seed=5, param_dir=param_dir)
         up, dn = _oracle_rows(orc, n)
