@@ -57,13 +57,16 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane((int)(th
 // 28 rounds cut by a footprint conflict or an event that cannot speculate,
 // 29 rounds cut by the key order; round selection: 30 the pops, 31 the
 // scalar copies after them; rescheduling batches: 32 pair predictions made
-// in them, 33 their 64-lane passes (32 / (64 * 33) is their lane occupancy).
+// in them, 33 their 64-lane passes (32 / (64 * 33) is their lane occupancy);
+// down-set work outside the batches: 34 single-pair updates predicted one
+// per lane, 35 full recomputes run one bead at a time, 36 events rescheduled
+// by the generic form.
 enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLeaf = 5, kPhRescan = 6,
        kPhSync = 7, kPhLists = 8, kPhEvents = 9, kPhGhost = 10, kPhRows = 11, kPhScan = 12, kPhNRebuild = 13,
        kPhNSync = 14, kPhLCells = 16, kPhLScan = 17, kPhLScatter = 18, kPhLCopy = 19, kPhLSearch = 20,
        kPhNRound = 21, kPhNPopped = 22, kPhNCommit = 23, kPhBMark = 24, kPhBCheck = 25, kPhBKeys = 26,
        kPhBTail = 27, kPhNCutMark = 28, kPhNCutKey = 29, kPhSelPop = 30, kPhSelCopy = 31, kPhNPred = 32,
-       kPhNPredPass = 33 };
+       kPhNPredPass = 33, kPhNSingle = 34, kPhNFull = 35, kPhNGeneric = 36 };
 #ifdef PDMD_PHASE_TIMERS
 #define PH_BEGIN(v) long long v = clock64()
 #define PH_END(S, k, v)                                                                                  \
@@ -81,6 +84,15 @@ enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLe
     do {                                                                                                  \
         if (threadIdx.x == 0) atomicAdd((unsigned long long*)&(S).st->phase[k], (unsigned long long)(n)); \
     } while (0)
+// a per-lane count, summed over the wavefront where it is added
+#define PH_LANE_DECL(v) int v = 0
+#define PH_LANE_INC(v) ++v
+#define PH_LANE_ADD(S, k, v)                                            \
+    do {                                                                \
+        int ph_s_ = v;                                                  \
+        for (int ph_o_ = 32; ph_o_ > 0; ph_o_ >>= 1) ph_s_ += __shfl_xor(ph_s_, ph_o_, 64); \
+        PH_ADDN(S, k, ph_s_);                                           \
+    } while (0)
 #else
 #define PH_BEGIN(v) \
     do {            \
@@ -96,6 +108,15 @@ enum { kPhSelect = 0, kPhPair = 1, kPhLoads = 2, kPhPred = 3, kPhDown = 4, kPhLe
     } while (0)
 #define PH_ADDN(S, k, n) \
     do {                 \
+    } while (0)
+#define PH_LANE_DECL(v) \
+    do {                \
+    } while (0)
+#define PH_LANE_INC(v) \
+    do {               \
+    } while (0)
+#define PH_LANE_ADD(S, k, v) \
+    do {                     \
     } while (0)
 #endif
 
@@ -1558,6 +1579,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
         }
         if (batch && lane == 0) atomicMin(&L.batch()->cut, 1);
         wave_sync_mem();
+        PH_COUNT(S, kPhNGeneric);
         reschedule_generic(S, T, L, c, P, extra_blk, xpulse_del);
         if (batch) {
             if (lane == 0) L.batch()->nbl[bk] = 0;
@@ -1895,6 +1917,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     // ---- the rest of each down bead's ops; full recomputes from the batch,
     //      others (not foreseen) cooperatively one at a time
     const bool have_full = spec && v_spec[myslot].off >= 0;
+    PH_LANE_DECL(ph_single);
     for (int guard = 0; guard < 64 * 9; ++guard) {
         while (first && (rem > 0 || pending)) {
             if (pending) {
@@ -1939,12 +1962,15 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
                         st_c = ty;
                         modified = true;
                     }
-                } else
+                } else {
                     single_update(T, rl, ol, bwl, p, opi ? P.ri : P.rj, tf, st_t, st_p, st_c, modified, status);
+                    PH_LANE_INC(ph_single);
+                }
             }
         }
         const unsigned long long pm = __ballot(pending);
         if (pm == 0) break;
+        PH_COUNT(S, kPhNFull);
         const int leader = __ffsll((long long)pm) - 1;
         const int lf_l = __shfl(l, leader, kW);
         const Rec rlf = shfl_rec(rl, leader);
@@ -1962,6 +1988,7 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
             pending = false;
         }
     }
+    PH_LANE_ADD(S, kPhNSingle, ph_single);
     // bead i's state lives in (ti, pi, ci) and is written with i's leaf
     const unsigned long long im = __ballot(is_i);
     if (im) {

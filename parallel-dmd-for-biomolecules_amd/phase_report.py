@@ -20,7 +20,7 @@ if HERE not in sys.path:
     sys.path.insert(0, HERE)
 import pdmd  # noqa: E402
 
-# slot: (name, parent slot or None); slots 13/14, 21-23, 28-29 and 32-33 are counts
+# slot: (name, parent slot or None); slots 13/14, 21-23, 28-29 and 32-36 are counts
 PHASES = {
     0: ("select", None), 30: ("select: pops", 0), 31: ("select: scalar copies", 0),
     1: ("pair event before rescheduling", None), 11: ("resched: list rows + work list", None),
@@ -83,7 +83,8 @@ def main():
         print(f"{name}: {args.events / (ms / 1e3):.0f} entries/s at W={args.waves}; rounds of two or more events "
               f"{d[21]}, popped {d[22]}, committed {d[23]}, cut by footprints {d[28]}, cut by key order {d[29]}; "
               f"list rebuilds {d[13]}; rescheduling batches: {d[32]} pair predictions in {d[33]} 64-lane passes "
-              f"(lane occupancy {d[32] / max(64 * d[33], 1):.3f}, wavefront 0)")
+              f"(lane occupancy {d[32] / max(64 * d[33], 1):.3f}, wavefront 0); outside the batches: {d[34]} "
+              f"single-pair updates, {d[35]} one-bead full recomputes, {d[36]} generic reschedules (wavefront 0)")
 
 
 if __name__ == "__main__":
