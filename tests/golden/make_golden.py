@@ -310,7 +310,17 @@ def reflected(sysd, seed, box=64.0, grid=2.0 ** -20, gap=4.5, nplane=16):
 
 
 def run_case(name, exe, seed, tstar=0.20, density=0.01, nsteps=200000, rng_seed=1, mirror=False, scatter=False,
-             reflect=False):
+             reflect=False, sizes=None):
+    """sizes: (L1, L2, NCH1, NCH2) of a checker built for other sizes; the
+    module's sizes are restored afterwards."""
+    global L1, L2, NCH1, NCH2
+    if sizes is not None:
+        saved = (L1, L2, NCH1, NCH2)
+        L1, L2, NCH1, NCH2 = sizes
+        try:
+            return run_case(name, exe, seed, tstar, density, nsteps, rng_seed, mirror, scatter, reflect)
+        finally:
+            L1, L2, NCH1, NCH2 = saved
     import pdmd
     rng = np.random.default_rng(seed)
     seq1 = [int(x) for x in rng.integers(9, 29, L1)]
@@ -349,7 +359,8 @@ def main():
     exe_n = os.path.join(OUT, "ref_harness_nohbs")
     exe_h = os.path.join(OUT, "ref_harness_noh")
     exe_c2 = os.path.join(OUT, "ref_harness_ch2")
-    if not all(os.path.exists(e) for e in (exe, exe_n, exe_h, exe_c2)):
+    exe_560 = os.path.join(OUT, "ref_harness_560")
+    if not all(os.path.exists(e) for e in (exe, exe_n, exe_h, exe_c2, exe_560)):
         raise SystemExit("build the checker first: make -C oracle/ref")
     run_case("ref_2species", exe, seed=4242)
     run_case("ref_2species_nohbs", exe_n, seed=4243)
@@ -368,6 +379,10 @@ def main():
     # mirror plane meet bit-identical times with two partners of one row
     # and rows' nbor.f order decides
     run_case("ref_2species_ptie", exe_c2, seed=50, tstar=0.20, density=0.01, nsteps=200000, reflect=True)
+    # 560 beads (species 1: 8 chains x 16 residues, as the 512-bead CPU
+    # configuration; species 2: 2 chains x 6), above the 256 beads up to
+    # which the engine builds lists without the cell sort
+    run_case("ref_560", exe_560, seed=4246, tstar=0.20, density=0.01, nsteps=200000, sizes=(16, 6, 8, 2))
 
 
 if __name__ == "__main__":

@@ -11,7 +11,8 @@ positions, velocities, calendar, identities, bptnr, extra_repuls -- at 1 and
 4 wavefronts per system (the two tie fixtures also at 2 and 8); the device's checkover.f / check_nc_int.f kernel
 against the reference's own messages on the final and two planted states;
 and the device's -Drunr restore (main.F90:240-321) of the final state
-against the reference's.
+against the reference's; the device's neighbour rows after set-up against
+the reference's own rows, in nbor.f's order.
 """
 import os
 
@@ -74,6 +75,29 @@ def _serial_loop(name, flags, waves):
     check_final_state(gs, fx, name)
     assert list(st.nevents[1:31]) == [int(x) for x in fx["loop_tally"]], name
     assert st.nupdates == fx["loop_updates"] and st.nforced == fx["loop_forced"], name
+
+
+@pytest.mark.parametrize("name,flags", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("waves", [1, 4])
+def test_device_rows_equal_reference_rows(name, flags, waves):
+    """Right after set-up, every bead's up and down rows on the device equal
+    the rows nbor.f built in the reference's own run, in their order (the
+    112-bead fixtures through the all-pairs stencil filter, the 560-bead one
+    through the cell sort)."""
+    fx = _fixture(name)
+    eng = pdmd.Engine(waves=waves)
+    try:
+        sid = eng.add_system(_sysd(fx, fx["sv0"]), seed=int(fx["rng_seed"]), canon=False, species=fx["species"],
+                             **flags)
+        gup, gdn = eng.lists(sid)
+    finally:
+        eng.close()
+    up = np.split(fx["nb_flat"], np.cumsum(fx["na_npt"])[:-1])
+    dn = np.split(fx["dn_flat"], np.cumsum(fx["nnabdn"])[:-1])
+    assert len(gup) == len(up)
+    for i in range(len(up)):
+        assert np.array_equal(gup[i], up[i]), f"{name}: up row of bead {i + 1}"
+        assert np.array_equal(gdn[i], dn[i]), f"{name}: down row of bead {i + 1}"
 
 
 @pytest.mark.parametrize("name,flags", CASES, ids=[c[0] for c in CASES])

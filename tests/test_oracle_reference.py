@@ -35,7 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # (fixture, build flags of the reference harness that wrote it)
 CASES = [("ref_2species", {}), ("ref_2species_nohbs", {"no_hbs": True}), ("ref_2species_noh", {"no_h": True}),
          ("ref_2species_hb", {}), ("ref_2species_tie", {}), ("ref_2species_ch2", {"chaptype2": True}),
-         ("ref_2species_ptie", {"chaptype2": True})]
+         ("ref_2species_ptie", {"chaptype2": True}), ("ref_560", {})]
 # coltypes of the worker's N-C resolution (main.F90:1487-1937, eventdyn.f)
 HB_TYPES = (14, 15, 16, 23, 24, 25, 26)
 LOG_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("type", "<i4"), ("t", "<f8")])
