@@ -5,7 +5,7 @@ in the same order (the device sorts each row into the order of nbor.f's
 cell traversal after its search), the longest row and the totals agree; after 20000 entries
 (several list rebuilds at the syncs) the rows agree again.  Systems of up
 to 256 beads build their lists with the all-pairs stencil filter, larger
-ones through the cell sort; both are covered, at 1 and 4 wavefronts."""
+ones through the cell sort; both are covered, at 1 and 4 wavefronts; random dense systems at 1, 4 and 8."""
 import numpy as np
 import pytest
 
@@ -75,6 +75,36 @@ def _lists_match(param_dir, chains, density, tstar, waves, cap):
         assert st.nupdates == orc.stats().nupdates
         up, dn = _oracle_rows(orc, n)
         assert round(st.avg_up * n) == up.sum() and round(st.avg_dn * n) == dn.sum()
+        _same_rows(eng, sid, orc, n)
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_dense_random_rows_and_events(param_dir, case):
+    """Random dense systems (rho* 0.08-0.3, 192-1280 beads, rows of up to
+    about 30 entries, many longer than the 20 a lane sorts alone) at 1, 4 and
+    8 wavefronts: rows equal to the oracle's in order after set-up and after
+    5000 entries, and those 5000 entries equal to the oracle's, times bit
+    for bit."""
+    rng = np.random.default_rng(123 + 1000 * case)
+    nch, L = int(rng.integers(3, 20)), int(rng.choice([8, 12, 16]))
+    dens, tstar = float(rng.uniform(0.08, 0.3)), float(rng.uniform(0.15, 0.5))
+    canon = bool(case % 2)
+    sysd = pdmd.generate([L] * nch, density=dens, tstar=tstar, seed=1000 + case)
+    eng = pdmd.Engine(waves=(1, 4, 8)[case % 3])
+    try:
+        sid = eng.add_system(sysd, seed=7 + case, canon=canon)
+        n = eng.nbeads[sid]
+        orc = Oracle(sysd, seed=7 + case, param_dir=param_dir, canon=canon)
+        _same_rows(eng, sid, orc, n)
+        eng.log_enable(sid, 5000)
+        eng.advance([sid], [5000])
+        g = eng.log_read(sid, 5000)
+        ol = orc.run(5000, log_cap=5000)
+        assert eng.stats(sid).status == 0
+        assert np.array_equal(g["i"], ol["i"]) and np.array_equal(g["j"], ol["j"])
+        assert np.array_equal(g["t"].view(np.int64), ol["t"].view(np.int64))
         _same_rows(eng, sid, orc, n)
     finally:
         eng.close()
