@@ -2,10 +2,11 @@
 restated, pinned by the reference's own lists in test_oracle_reference.py):
 right after set-up every bead's up and down rows hold the oracle's partners
 in the same order (the device sorts each row into the order of nbor.f's
-cell traversal after its search), the longest row and the totals agree; after 20000 entries
-(several list rebuilds at the syncs) the rows agree again.  Systems of up
-to 256 beads build their lists with the all-pairs stencil filter, larger
-ones through the cell sort; both are covered, at 1 and 4 wavefronts; random dense systems at 1, 4 and 8."""
+cell traversal after its search), the longest row and the totals agree;
+after 20000 entries (several list rebuilds at the syncs) the rows agree
+again.  Systems of up to 256 beads build their lists with the all-pairs
+stencil filter, larger ones through the cell sort; both are covered, at 1
+and 4 wavefronts, and random dense systems at 1, 4 and 8."""
 import numpy as np
 import pytest
 
