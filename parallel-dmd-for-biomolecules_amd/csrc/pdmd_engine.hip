@@ -1804,8 +1804,8 @@ __device__ void reschedule(const SysDev&, const DevTables&, const Lds&, Sc&, con
     const int ntot = sing_fit ? npairs + nsing : npairs;
     wave_sync();
     PH_END(S, kPhLoads, tph);
-    // ---- the prediction batch (round 3: the batched beads' partners, then
-    //      the single-pair predictions)
+    // ---- the prediction batch: the candidates of i and j, the batched
+    //      beads' partners, then the single-pair predictions
     PH_ADDN(S, kPhNPred, ntot);
     PH_ADDN(S, kPhNPredPass, (ntot + kW - 1) / kW);
     for (int base = 0; base < ntot; base += kW) {
